@@ -1,0 +1,90 @@
+"""Builds the native parts of faces_amd in-tree.
+
+  faces_amd/lib/libfaces_amd.so   HIP kernels + C ABI (include/faces_amd.h), gfx950
+  faces_amd/lib/faces_run         C++ stdin driver over the C ABI
+
+Run ``python faces_amd/build.py`` (or ``__graft_entry__.build()``).  The
+commands are plain hipcc/g++ invocations; every HIP translation unit is built
+for gfx950 only and without packed-FP32 instructions.
+"""
+from __future__ import annotations
+
+import os
+import shlex
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIBDIR = os.path.join(PKG, "lib")
+INCLUDE = os.path.join(ROOT, "include")
+
+ARCH = "gfx950"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_SOURCES = ["faces_plan.hip", "faces_batch.hip", "faces_serve.cpp"]
+
+COMMON_FLAGS = [
+    "-std=c++17",
+    "-O3",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Wall",
+    "-Wno-unused-function",
+    f"-I{INCLUDE}",
+]
+DEVICE_FLAGS = [
+    f"--offload-arch={ARCH}",
+    # Packed FP32 forms are kept out of every kernel.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(shlex.quote(c) for c in cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_library(verbose: bool = True) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libfaces_amd.so")
+    tmp = out + ".tmp"
+    srcs = []
+    for s in HIP_SOURCES:
+        p = os.path.join(CSRC, s)
+        if not os.path.exists(p):
+            continue
+        if s.endswith(".cpp"):
+            srcs += ["-x", "hip", p]
+        else:
+            srcs.append(p)
+    cmd = [HIPCC, *DEVICE_FLAGS, *COMMON_FLAGS, "-shared", "-o", tmp, *srcs, "-lpthread"]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_driver() -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "faces_run")
+    src = os.path.join(CSRC, "faces_run.cpp")
+    cmd = [
+        "g++", "-std=c++17", "-O2", "-Wall", f"-I{INCLUDE}", "-D__HIP_PLATFORM_AMD__",
+        "-I/opt/rocm/include", src, "-o", out,
+        f"-L{LIBDIR}", "-lfaces_amd", "-Wl,-rpath,$ORIGIN",
+        "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib",
+    ]
+    _run(cmd)
+    return out
+
+
+def build_all() -> None:
+    build_library()
+    if os.path.exists(os.path.join(CSRC, "faces_run.cpp")):
+        build_driver()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

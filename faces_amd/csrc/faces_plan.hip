@@ -1,0 +1,509 @@
+// faces_plan.hip -- the plain exchange + DSS step behind the C ABI
+// (replaces class Faces, gpu/v-26-messages/Faces.{h,cc}).
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "faces_common.hpp"
+#include "faces_kernels.hpp"
+
+namespace faces {
+
+static thread_local std::string g_last_error;
+
+int fail(int status, const char *fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return status;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+  const int st = (e == hipErrorOutOfMemory) ? FACES_ERR_NOMEM : FACES_ERR_HIP;
+  return fail(st, "%s: %s (%s)", what, hipGetErrorName(e), hipGetErrorString(e));
+}
+
+int device_cus()
+{
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cached[dev] == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    cached[dev] = cus;
+  }
+  return cached[dev];
+}
+
+}  // namespace faces
+
+using namespace faces;
+
+struct faces_plan {
+  int rank, lx, ly, lz, kx, ky, kz;
+  int mx, my, mz, n;
+  int boundary;
+  long ps;
+  int nbr[26];   // neighbour rank per slot, -1 = none
+  int mask;      // face-neighbour bits x-,x+,y-,y+,z-,z+
+  int path;      // FACES_PATH_*
+  int block, seg_len, xcd_remap;
+  // message arena: send slots then receive slots
+  double *arena;
+  long arena_doubles;
+  long slot_cnt[26];
+  Slots send, recv;
+};
+
+namespace {
+
+constexpr int kMaxN = 64;
+
+int neighbor_of(const faces_plan *p, int dx, int dy, int dz)
+{
+  // gpu/v-26-messages/Faces.cc:92-103, with the periodic rank grid wrapping.
+  int kx = p->kx + dx, ky = p->ky + dy, kz = p->kz + dz;
+  if (p->boundary == FACES_BOUNDARY_PERIODIC) {
+    kx = (kx + p->lx) % p->lx;
+    ky = (ky + p->ly) % p->ly;
+    kz = (kz + p->lz) % p->lz;
+  } else {
+    if (kx < 0 || kx >= p->lx || ky < 0 || ky >= p->ly || kz < 0 || kz >= p->lz) return -1;
+  }
+  return (kz * p->ly + ky) * p->lx + kx;
+}
+
+void slot_dir(int k, int &dx, int &dy, int &dz)
+{
+  dx = dy = dz = 0;
+  if (k < 6) {
+    const int s = (k & 1) ? 1 : -1;
+    if (k < 2) dx = s; else if (k < 4) dy = s; else dz = s;
+  } else if (k < 18) {
+    const int f = (k - 6) / 4, e = (k - 6) % 4;
+    const int a = (e & 1) ? 1 : -1, b = (e >> 1) ? 1 : -1;
+    if (f == 0) { dy = a; dz = b; }        // iedge_[0..3]  (0,+-1,+-1)
+    else if (f == 1) { dx = a; dz = b; }   // iedge_[4..7]  (+-1,0,+-1)
+    else { dx = a; dy = b; }               // iedge_[8..11] (+-1,+-1,0)
+  } else {
+    const int c = k - 18;
+    dx = (c & 1) ? 1 : -1;
+    dy = (c & 2) ? 1 : -1;
+    dz = (c & 4) ? 1 : -1;
+  }
+}
+
+Geom make_geom(const faces_plan *p, double *u)
+{
+  Geom g;
+  g.u = u;
+  g.n = p->n;
+  g.mx = p->mx;
+  g.my = p->my;
+  g.mz = p->mz;
+  g.ys = (long)p->n * p->n * p->n * p->mx;
+  g.ps = p->ps;
+  return g;
+}
+
+Bound bound_wrap()
+{
+  Bound b;
+  b.x = b.y = b.z = Axis{1, 0, 0};
+  return b;
+}
+
+Bound bound_recv(int mask)
+{
+  Bound b;
+  b.x = Axis{0, mask & 1, (mask >> 1) & 1};
+  b.y = Axis{0, (mask >> 2) & 1, (mask >> 3) & 1};
+  b.z = Axis{0, (mask >> 4) & 1, (mask >> 5) & 1};
+  return b;
+}
+
+SurfDesc surf_desc(const faces_plan *p, const Bound &b)
+{
+  SurfDesc s;
+  const int K = p->n - 1;
+  s.Gx = p->mx * K + (b.x.wrap ? 0 : 1);
+  s.Gy = p->my * K + (b.y.wrap ? 0 : 1);
+  s.Gz = p->mz * K + (b.z.wrap ? 0 : 1);
+  s.nsx = b.x.wrap ? 1 : 2;
+  s.nsy = b.y.wrap ? 1 : 2;
+  s.nsz = b.z.wrap ? 1 : 2;
+  s.cz = (long)s.nsz * s.Gx * s.Gy;
+  s.cy = (long)s.nsy * s.Gx * (s.Gz - s.nsz);
+  s.cx = (long)s.nsx * (s.Gy - s.nsy) * (s.Gz - s.nsz);
+  return s;
+}
+
+int grid_for(long work, int block)
+{
+  const long cap = (long)device_cus() * 8;
+  long g = (work + block - 1) / block;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+int launch_box(const faces_plan *p, double *u, bool skip_surface, hipStream_t st)
+{
+  Geom G = make_geom(p, u);
+  RowItems R;
+  R.my = p->my;
+  R.mz = p->mz;
+  R.seg_len = (p->seg_len <= 0 || p->seg_len > p->mx) ? p->mx : p->seg_len;
+  R.segs = (p->mx + R.seg_len - 1) / R.seg_len;
+  R.nitems = p->my * p->mz * R.segs;
+  R.xcd_remap = p->xcd_remap;
+  const int nb = R.xcd_remap ? 8 * ((R.nitems + 7) / 8) : R.nitems;
+  const dim3 grid(nb), blk(p->block);
+#define FACES_LAUNCH_BOX(NN)                                              \
+  if (skip_surface) k_box<NN, true><<<grid, blk, 0, st>>>(G, R);          \
+  else k_box<NN, false><<<grid, blk, 0, st>>>(G, R);
+  FACES_DISPATCH_N(p->n, FACES_LAUNCH_BOX)
+#undef FACES_LAUNCH_BOX
+  FACES_HIP(hipGetLastError());
+  return FACES_OK;
+}
+
+int launch_surface(const faces_plan *p, double *u, const Bound &b, hipStream_t st)
+{
+  Geom G = make_geom(p, u);
+  SurfDesc S = surf_desc(p, b);
+  const long total = S.cz + S.cy + S.cx;
+  if (total == 0) return FACES_OK;
+  const int blk = 256;
+  const int grid = grid_for(total, blk);
+  const double *recv = p->arena;
+  Slots sl = p->recv;
+#define FACES_LAUNCH_SURF(NN) k_surface<NN><<<grid, blk, 0, st>>>(G, b, recv, sl, S);
+  FACES_DISPATCH_N(p->n, FACES_LAUNCH_SURF)
+#undef FACES_LAUNCH_SURF
+  FACES_HIP(hipGetLastError());
+  return FACES_OK;
+}
+
+int launch_pack(const faces_plan *p, const double *u, bool direct, hipStream_t st)
+{
+  Geom G = make_geom(p, const_cast<double *>(u));
+  PackDesc P;
+  P.src_off[0] = 0;
+  for (int k = 0; k < 26; k++) P.src_off[k + 1] = P.src_off[k] + p->slot_cnt[k];
+  for (int k = 0; k < 26; k++) {
+    if (!direct) {
+      P.dst[k] = p->send.off[k];
+    } else {
+      const int o = opposite_slot(k);
+      P.dst[k] = (p->nbr[o] >= 0) ? p->recv.off[o] : -1;
+    }
+  }
+  const int blk = 256;
+  const int grid = grid_for(P.src_off[26], blk);
+#define FACES_LAUNCH_PACK(NN) k_pack<NN><<<grid, blk, 0, st>>>(G, P, p->arena);
+  FACES_DISPATCH_N(p->n, FACES_LAUNCH_PACK)
+#undef FACES_LAUNCH_PACK
+  FACES_HIP(hipGetLastError());
+  return FACES_OK;
+}
+
+int launch_exchange(const faces_plan *p, hipStream_t st)
+{
+  XchgDesc X;
+  int ns = 0;
+  X.idx_off[0] = 0;
+  for (int k = 0; k < 26; k++) {
+    if (p->nbr[k] < 0) continue;
+    X.src[ns] = p->send.off[opposite_slot(k)];
+    X.dst[ns] = p->recv.off[k];
+    X.idx_off[ns + 1] = X.idx_off[ns] + p->slot_cnt[k];
+    ns++;
+  }
+  X.nslots = ns;
+  if (ns == 0) return FACES_OK;
+  const int blk = 256;
+  const int grid = grid_for(X.idx_off[ns], blk);
+  k_exchange<<<grid, blk, 0, st>>>(X, p->arena);
+  FACES_HIP(hipGetLastError());
+  return FACES_OK;
+}
+
+int check_plan(const faces_plan *p, const void *u)
+{
+  if (!p) return fail(FACES_ERR_ARG, "null plan");
+  if (!u) return fail(FACES_ERR_ARG, "null field pointer");
+  return FACES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *faces_strerror(int status)
+{
+  switch (status) {
+    case FACES_OK: return "success";
+    case FACES_ERR_ARG: return "invalid argument";
+    case FACES_ERR_HIP: return "HIP runtime error";
+    case FACES_ERR_NOMEM: return "out of memory";
+    case FACES_ERR_STATE: return "invalid state";
+    case FACES_ERR_ADMISSION: return "request exceeds footprint cap";
+    case FACES_ERR_NODEV: return "no HIP device";
+    case FACES_ERR_TIMEOUT: return "timed out";
+    default: return "unknown status";
+  }
+}
+
+const char *faces_last_error(void) { return g_last_error.c_str(); }
+
+const char *faces_version(void) { return "faces_amd 0.1 (gfx950)"; }
+
+long faces_field_doubles(int n, int mx, int my, int mz, long plane_stride)
+{
+  if (n < 1 || mx < 1 || my < 1 || mz < 1) return -1;
+  const long dense = (long)n * n * n * mx * my;
+  const long ps = plane_stride > 0 ? plane_stride : dense;
+  return ps * (long)(mz - 1) + dense;
+}
+
+int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx, int my, int mz,
+                      int n, int boundary, long plane_stride)
+{
+  if (!out) return fail(FACES_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (n < 2 || n > kMaxN) return fail(FACES_ERR_ARG, "element size n=%d outside [2,%d]", n, kMaxN);
+  if (mx < 1 || my < 1 || mz < 1)
+    return fail(FACES_ERR_ARG, "element counts must be positive (%d,%d,%d)", mx, my, mz);
+  if (lx < 1 || ly < 1 || lz < 1)
+    return fail(FACES_ERR_ARG, "rank grid must be positive (%d,%d,%d)", lx, ly, lz);
+  if ((long)lx * ly * lz > (1L << 30)) return fail(FACES_ERR_ARG, "rank grid too large");
+  if (rank < 0 || rank >= lx * ly * lz)
+    return fail(FACES_ERR_ARG, "rank %d outside rank grid of %d", rank, lx * ly * lz);
+  if (boundary != FACES_BOUNDARY_OPEN && boundary != FACES_BOUNDARY_PERIODIC)
+    return fail(FACES_ERR_ARG, "unknown boundary %d", boundary);
+  const long dense = (long)n * n * n * mx * my;
+  if (plane_stride != 0 && plane_stride < dense)
+    return fail(FACES_ERR_ARG, "plane stride %ld below dense plane %ld", plane_stride, dense);
+  // Row-local node indices are int; global node counts per axis must fit too.
+  if ((long)mx * (n - 1) + 1 > (1L << 30) || (long)my * (n - 1) + 1 > (1L << 30) ||
+      (long)mz * (n - 1) + 1 > (1L << 30) || (long)mx * (n - 1) * (n - 1) * (n - 1) > (1L << 30))
+    return fail(FACES_ERR_ARG, "grid too large for one plan");
+
+  faces_plan *p = new (std::nothrow) faces_plan();
+  if (!p) return fail(FACES_ERR_NOMEM, "plan allocation");
+  p->rank = rank;
+  p->lx = lx; p->ly = ly; p->lz = lz;
+  // gpu/v-26-messages/Faces.cc:33-35
+  p->kz = rank / (lx * ly);
+  p->ky = (rank - p->kz * lx * ly) / lx;
+  p->kx = rank - (p->kz * ly + p->ky) * lx;
+  p->mx = mx; p->my = my; p->mz = mz; p->n = n;
+  p->boundary = boundary;
+  p->ps = plane_stride > 0 ? plane_stride : dense;
+  p->mask = 0;
+  for (int k = 0; k < 26; k++) {
+    int dx, dy, dz;
+    slot_dir(k, dx, dy, dz);
+    p->nbr[k] = neighbor_of(p, dx, dy, dz);
+  }
+  for (int k = 0; k < 6; k++)
+    if (p->nbr[k] >= 0) p->mask |= 1 << k;
+  p->path = FACES_PATH_AUTO;
+  p->block = 256;
+  p->seg_len = 0;
+  p->xcd_remap = 1;
+
+  // Message sizes, gpu/v-26-messages/Faces.cc:44-47, 64-66.
+  const long nn = (long)n * n;
+  for (int s = 0; s < 2; s++) {
+    p->slot_cnt[0 + s] = nn * my * mz;
+    p->slot_cnt[2 + s] = nn * mx * mz;
+    p->slot_cnt[4 + s] = nn * mx * my;
+  }
+  for (int e = 0; e < 4; e++) {
+    p->slot_cnt[6 + e] = (long)n * mx;
+    p->slot_cnt[10 + e] = (long)n * my;
+    p->slot_cnt[14 + e] = (long)n * mz;
+  }
+  for (int c = 0; c < 8; c++) p->slot_cnt[18 + c] = 1;
+  // Slots start on 64-double (512 B) boundaries.
+  long off = 0;
+  for (int side = 0; side < 2; side++) {
+    Slots &sl = side ? p->recv : p->send;
+    for (int k = 0; k < 26; k++) {
+      sl.off[k] = off;
+      off += (p->slot_cnt[k] + 63) / 64 * 64;
+    }
+    sl.off[26] = off;
+  }
+  p->arena_doubles = off;
+  hipError_t e = hipMalloc(&p->arena, sizeof(double) * (size_t)off);
+  if (e != hipSuccess) {
+    delete p;
+    return hip_fail(e, "hipMalloc(message arena)");
+  }
+  // Receives from a missing neighbour leave zeros (gpu/DArray.h:81).
+  e = hipMemset(p->arena, 0, sizeof(double) * (size_t)off);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(p->arena);
+    delete p;
+    return hip_fail(e, "hipMemset(message arena)");
+  }
+  *out = p;
+  return FACES_OK;
+}
+
+int faces_plan_destroy(faces_plan *p)
+{
+  if (!p) return FACES_OK;
+  hipError_t e = hipSuccess;
+  if (p->arena) e = hipFree(p->arena);
+  delete p;
+  if (e != hipSuccess) return hip_fail(e, "hipFree(message arena)");
+  return FACES_OK;
+}
+
+int faces_plan_set_path(faces_plan *p, int path)
+{
+  if (!p) return fail(FACES_ERR_ARG, "null plan");
+  if (path < FACES_PATH_AUTO || path > FACES_PATH_STAGED_DIRECT)
+    return fail(FACES_ERR_ARG, "unknown path %d", path);
+  if (path == FACES_PATH_FUSED && p->boundary == FACES_BOUNDARY_OPEN && p->mask != 0)
+    return fail(FACES_ERR_ARG, "fused path needs a periodic grid or a rank without neighbours");
+  p->path = path;
+  return FACES_OK;
+}
+
+int faces_plan_set_tuning(faces_plan *p, int block, int seg_len, int xcd_remap)
+{
+  if (!p) return fail(FACES_ERR_ARG, "null plan");
+  if (block < 64 || block > 1024 || block % 64)
+    return fail(FACES_ERR_ARG, "block %d not a multiple of 64 in [64,1024]", block);
+  if (seg_len < 0) return fail(FACES_ERR_ARG, "negative segment length");
+  p->block = block;
+  p->seg_len = seg_len;
+  p->xcd_remap = xcd_remap ? 1 : 0;
+  return FACES_OK;
+}
+
+int faces_plan_neighbor(const faces_plan *p, int slot, int *rank)
+{
+  if (!p || !rank) return fail(FACES_ERR_ARG, "null argument");
+  if (slot < 0 || slot >= 26) return fail(FACES_ERR_ARG, "slot %d outside [0,26)", slot);
+  *rank = p->nbr[slot];
+  return FACES_OK;
+}
+
+int faces_plan_message(faces_plan *p, int slot, int recv, double **dptr, long *count)
+{
+  if (!p || !dptr || !count) return fail(FACES_ERR_ARG, "null argument");
+  if (slot < 0 || slot >= 26) return fail(FACES_ERR_ARG, "slot %d outside [0,26)", slot);
+  const Slots &sl = recv ? p->recv : p->send;
+  *dptr = p->arena + sl.off[slot];
+  *count = p->slot_cnt[slot];
+  return FACES_OK;
+}
+
+int faces_pack(faces_plan *p, const double *u, void *stream)
+{
+  int st = check_plan(p, u);
+  if (st) return st;
+  return launch_pack(p, u, false, (hipStream_t)stream);
+}
+
+int faces_exchange(faces_plan *p, void *stream)
+{
+  if (!p) return fail(FACES_ERR_ARG, "null plan");
+  return launch_exchange(p, (hipStream_t)stream);
+}
+
+int faces_inner(faces_plan *p, double *u, void *stream)
+{
+  int st = check_plan(p, u);
+  if (st) return st;
+  return launch_box(p, u, true, (hipStream_t)stream);
+}
+
+int faces_unpack(faces_plan *p, double *u, void *stream)
+{
+  int st = check_plan(p, u);
+  if (st) return st;
+  return launch_surface(p, u, bound_recv(p->mask), (hipStream_t)stream);
+}
+
+int faces_share(faces_plan *p, double *u, int compute, void *stream)
+{
+  int st = check_plan(p, u);
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  int path = p->path;
+  if (path == FACES_PATH_AUTO)
+    path = (p->boundary == FACES_BOUNDARY_PERIODIC || p->mask == 0) ? FACES_PATH_FUSED
+                                                                    : FACES_PATH_STAGED_DIRECT;
+  if (path == FACES_PATH_FUSED) {
+    if (p->boundary == FACES_BOUNDARY_PERIODIC) {
+      // One launch: every node with its wrap partners, in place.
+      if (compute) return launch_box(p, u, false, s);
+      return launch_surface(p, u, bound_wrap(), s);
+    }
+    // Open rank without neighbours: interior nodes, then the rank surface.
+    if (compute) {
+      st = launch_box(p, u, true, s);
+      if (st) return st;
+    }
+    return launch_surface(p, u, bound_recv(0), s);
+  }
+  // Staged: pack -> (exchange) -> inner -> unpack.  Inner and unpack touch
+  // disjoint node sets, and pack reads only rank-surface copies, which inner
+  // never writes (gpu/v-26-messages/Faces.cc:297-380 runs inner concurrently
+  // with pack for the same reason).
+  st = launch_pack(p, u, path == FACES_PATH_STAGED_DIRECT, s);
+  if (st) return st;
+  if (path == FACES_PATH_STAGED) {
+    st = launch_exchange(p, s);
+    if (st) return st;
+  }
+  if (compute) {
+    st = launch_box(p, u, true, s);
+    if (st) return st;
+  }
+  return launch_surface(p, u, bound_recv(p->mask), s);
+}
+
+int faces_fill(double *u, int n, int mx, int my, int mz, long plane_stride, int kind, uint64_t seed,
+               void *stream)
+{
+  if (!u) return fail(FACES_ERR_ARG, "null field pointer");
+  if (n < 1 || mx < 1 || my < 1 || mz < 1) return fail(FACES_ERR_ARG, "bad shape");
+  if (kind < 0 || kind > 2) return fail(FACES_ERR_ARG, "unknown fill kind %d", kind);
+  Geom G;
+  G.u = u;
+  G.n = n;
+  G.mx = mx;
+  G.my = my;
+  G.mz = mz;
+  G.ys = (long)n * n * n * mx;
+  const long dense = G.ys * my;
+  if (plane_stride != 0 && plane_stride < dense) return fail(FACES_ERR_ARG, "bad plane stride");
+  G.ps = plane_stride > 0 ? plane_stride : dense;
+  const long total = dense * mz;
+  const int blk = 256;
+  const int grid = grid_for(total, blk);
+  k_fill<<<grid, blk, 0, (hipStream_t)stream>>>(G, kind, seed);
+  FACES_HIP(hipGetLastError());
+  return FACES_OK;
+}
+
+}  // extern "C"

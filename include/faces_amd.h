@@ -1,0 +1,188 @@
+/*
+ * faces_amd.h -- C ABI of the MI355X (gfx950) Faces exchange + DSS library.
+ *
+ * The library replaces the step API of trey-ornl/faces, class Faces
+ * (gpu/v-26-messages/Faces.h:6-39, Faces.cc:10-678), with plain C entry points:
+ * device pointers, sizes and an opaque HIP stream handle (void *; NULL is the
+ * null stream).  No entry point exits the process: each returns FACES_OK or a
+ * negative status, and faces_strerror / faces_last_error give the text.
+ *
+ * Field layout (gpu/DArray.h:36-38): u(ix,iy,iz,jx,jy,jz) at
+ *   ix + n*(iy + n*(iz + n*(jx + mx*jy))) + jz*plane_stride
+ * with plane_stride = n^3*mx*my for a dense field (pass 0), or the
+ * reference's page-padded stride (gpu/DArray.h:95-99).
+ *
+ * Semantics: every copy of a shared node becomes the sum of all its copies,
+ * grouped x, then y, then z (the order of gpu/Mugs.cc:136-203).
+ *   FACES_BOUNDARY_OPEN      neighbours outside the rank grid do not exist, as
+ *                            in the reference's neighbor() (Faces.cc:92-103);
+ *                            existing neighbours are stood in for by this
+ *                            rank's own opposite boundary (self halo).
+ *   FACES_BOUNDARY_PERIODIC  the rank grid wraps; on one device every
+ *                            neighbour is stood in for by the self halo, which
+ *                            makes the field periodic.
+ */
+#ifndef FACES_AMD_H
+#define FACES_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define FACES_OK 0
+#define FACES_ERR_ARG (-1)       /* invalid argument */
+#define FACES_ERR_HIP (-2)       /* HIP runtime error */
+#define FACES_ERR_NOMEM (-3)     /* device or host allocation failed */
+#define FACES_ERR_STATE (-4)     /* call not valid in the object's state */
+#define FACES_ERR_ADMISSION (-5) /* request exceeds the server's footprint cap */
+#define FACES_ERR_NODEV (-6)     /* no HIP device visible */
+#define FACES_ERR_TIMEOUT (-7)   /* wait ended before the work completed */
+
+#define FACES_BOUNDARY_OPEN 0
+#define FACES_BOUNDARY_PERIODIC 1
+
+/* Execution paths of faces_share. */
+#define FACES_PATH_AUTO 0          /* fused when possible, else staged */
+#define FACES_PATH_FUSED 1         /* gather-sum-scatter in place, one launch */
+#define FACES_PATH_STAGED 2        /* pack -> exchange copy -> inner + unpack */
+#define FACES_PATH_STAGED_DIRECT 3 /* pack straight into receive slots */
+
+/* Field generators (faces_fill). */
+#define FACES_FILL_INT 0  /* integer-valued, uniform in [0,1023] */
+#define FACES_FILL_RAND 1 /* uniform in [-1,1) */
+#define FACES_FILL_REF 2  /* reference init, gpu/main.cc:10-19 */
+
+/* Text of a status code. */
+const char *faces_strerror(int status);
+/* Detail of the last failure on the calling thread ("" if none). */
+const char *faces_last_error(void);
+/* Library version string. */
+const char *faces_version(void);
+
+/* ---- plain step: replaces class Faces ------------------------------------ */
+typedef struct faces_plan faces_plan;
+
+/* Faces::Faces(rank,lx,ly,lz,mx,my,mz,n) -- gpu/v-26-messages/Faces.cc:10-82.
+ * Builds the neighbour table and message sizes, allocates the 26 send and 26
+ * receive slots once, and picks the launch shapes.  plane_stride 0 = dense. */
+int faces_plan_create(faces_plan **plan, int rank, int lx, int ly, int lz, int mx, int my, int mz,
+                      int n, int boundary, long plane_stride);
+
+/* Faces::~Faces -- gpu/v-26-messages/Faces.cc:84-90. */
+int faces_plan_destroy(faces_plan *plan);
+
+/* Faces::share(u, compute) -- gpu/v-26-messages/Faces.cc:111-678.
+ * Stream-ordered: returns once the work is enqueued on `stream` (the
+ * reference returns after synchronising, Faces.cc:675-677; call
+ * hipStreamSynchronize for that).  compute = 0 skips the rank-interior sums
+ * and sums only nodes on the rank surface (Faces.cc:299, gpu/main.cc:157-161).
+ * Allocates nothing; may be captured in a HIP graph. */
+int faces_share(faces_plan *plan, double *u, int compute, void *stream);
+
+/* Selects the execution path (FACES_PATH_*). */
+int faces_plan_set_path(faces_plan *plan, int path);
+
+/* Launch shape of the main sum kernel: threads per block (64..1024, multiple
+ * of 64), elements per row segment (0 = whole row), XCD-contiguous work order
+ * (0/1).  No counterpart in the reference (gpu/gpu.h:73-78 fixes MAXTHREADS
+ * at build time). */
+int faces_plan_set_tuning(faces_plan *plan, int block, int seg_len, int xcd_remap);
+
+/* Neighbour rank of message slot 0..25, or -1 for none (MPI_PROC_NULL) --
+ * Faces::neighbor, gpu/v-26-messages/Faces.cc:92-103. */
+int faces_plan_neighbor(const faces_plan *plan, int slot, int *rank);
+
+/* Device pointer and length (doubles) of message slot `slot` of the send
+ * (recv = 0) or receive (recv = 1) side -- the buffers the reference posts to
+ * MPI_Isend / MPI_Irecv (Faces.cc:144-171, 386-413). */
+int faces_plan_message(faces_plan *plan, int slot, int recv, double **dptr, long *count);
+
+/* The staged step in pieces, for an external transport (e.g. one process per
+ * GPU).  pack: Faces.cc:173-295; exchange: self-halo copy recv[d] <-
+ * send[opposite(d)] for every existing neighbour (stand-in for Faces.cc:
+ * 144-171, 386-417); inner: Faces.cc:297-380; unpack: Faces.cc:419-671. */
+int faces_pack(faces_plan *plan, const double *u, void *stream);
+int faces_exchange(faces_plan *plan, void *stream);
+int faces_inner(faces_plan *plan, double *u, void *stream);
+int faces_unpack(faces_plan *plan, double *u, void *stream);
+
+/* ---- field helpers ------------------------------------------------------- */
+
+/* Fills a field with one of the FACES_FILL_* generators (gpu/main.cc:10-19 for
+ * FACES_FILL_REF; splitmix64 of seed ^ index*golden for the others). */
+int faces_fill(double *u, int n, int mx, int my, int mz, long plane_stride, int kind, uint64_t seed,
+               void *stream);
+
+/* Doubles spanned by a field (including plane padding). */
+long faces_field_doubles(int n, int mx, int my, int mz, long plane_stride);
+
+/* ---- batched step: many independent grids, one launch -------------------- */
+/* No counterpart in the reference (north-star extension). */
+typedef struct faces_batch faces_batch;
+
+/* Describes `ngrids` grids (dense fields).  boundary as above; every grid is
+ * one rank 1x1x1. */
+int faces_batch_create(faces_batch **batch, int ngrids, const int *n, const int *mx, const int *my,
+                       const int *mz, int boundary);
+int faces_batch_destroy(faces_batch *batch);
+
+/* Binds the device field of every grid (host array of `ngrids` device
+ * pointers).  Rebuilds the captured graph on the next share. */
+int faces_batch_set_fields(faces_batch *batch, double *const *u);
+
+/* One step over all grids: one persistent work-list launch, replayed from a
+ * captured HIP graph. */
+int faces_batch_share(faces_batch *batch, int compute, void *stream);
+
+/* Same step, one launch per grid (the baseline the batched form is measured
+ * against). */
+int faces_batch_share_per_grid(faces_batch *batch, int compute, void *stream);
+
+/* Launch shape: threads per block, elements per work item (0 = whole row),
+ * persistent blocks per CU (0 = one block per work item). */
+int faces_batch_set_tuning(faces_batch *batch, int block, int seg_len, int blocks_per_cu);
+
+/* Times every candidate launch shape on the bound fields (reps replays each,
+ * fields are modified) and keeps the fastest.  out[0..3] = block, seg_len,
+ * blocks_per_cu, microseconds per step.  Not graph-capturable. */
+int faces_batch_tune(faces_batch *batch, int reps, void *stream, double *out);
+
+/* Total bytes of the bound fields and number of work items. */
+int faces_batch_info(const faces_batch *batch, double *bytes, long *items);
+
+/* ---- served step: continuous batching of requests ------------------------ */
+/* No counterpart in the reference (north-star extension). */
+typedef struct faces_server faces_server;
+
+/* footprint_cap: bytes of request fields admitted at once; max_batch: most
+ * requests per launch; nstreams: launches that may overlap on the device. */
+int faces_server_create(faces_server **srv, double footprint_cap, int max_batch, int nstreams);
+int faces_server_destroy(faces_server *srv);
+
+/* Enqueues a request on the dense field u (device, resident until done).
+ * Returns FACES_ERR_ADMISSION if the field alone exceeds the cap. */
+int faces_server_submit(faces_server *srv, int n, int mx, int my, int mz, int boundary, int compute,
+                        double *u, uint64_t *ticket);
+
+/* Waits for a request; latency_us = completion - submission. */
+int faces_server_wait(faces_server *srv, uint64_t ticket, double timeout_s, double *latency_us);
+
+/* Replays an arrival trace: request i is submitted at arrival_us[i] after the
+ * start; fills latency_us[i] (arrival -> completion).  stats[0..5] = wall
+ * seconds, requests per second, p50 us, p99 us, max us, launches. */
+int faces_server_run_trace(faces_server *srv, int nreq, const double *arrival_us, const int *n,
+                           const int *mx, const int *my, const int *mz, int boundary, int compute,
+                           double *const *u, double *latency_us, double *stats);
+
+/* stats[0..3] = admitted, completed, launches, peak admitted bytes. */
+int faces_server_stats(const faces_server *srv, double *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FACES_AMD_H */

@@ -1,0 +1,258 @@
+"""GPU parity of the plain step (faces_share and its staged pieces) against the
+oracle, through the C ABI.  Bar: bit-exact -- the kernels evaluate the same
+x-then-y-then-z tree as the oracle, so random FP64 fields match bit for bit,
+not only integer-valued ones."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import _brute as brute
+
+pytestmark = pytest.mark.gpu
+
+import faces_amd as fa  # noqa: E402
+
+
+def to_dev(torch, u: np.ndarray):
+    return torch.from_numpy(u).to("cuda")
+
+
+def to_host(torch, d) -> np.ndarray:
+    torch.cuda.synchronize()
+    return d.cpu().numpy()
+
+
+def run_gpu(torch, u, shape, boundary=fa.PERIODIC, compute=True, path=fa.PATH_AUTO, ps=0,
+            rank=0, lgrid=(1, 1, 1), nshare=1, tuning=None):
+    n, mx, my, mz = shape
+    d = to_dev(torch, u)
+    f = fa.Faces(rank, *lgrid, mx, my, mz, n, boundary=boundary, plane_stride=ps)
+    f.set_path(path)
+    if tuning:
+        f.set_tuning(*tuning)
+    for _ in range(nshare):
+        f.share(d, compute=compute)
+    out = to_host(torch, d)
+    f.close()
+    return out
+
+
+def run_oracle(u, shape, periodic=True, compute=True, ps=0, nshare=1):
+    v = u.copy()
+    for _ in range(nshare):
+        oracle.share_sweep(v, *shape, periodic=periodic, compute=compute, ps=ps)
+    return v
+
+
+def assert_same(a, b, shape, ps=0):
+    r = oracle.compare(a, b, *shape, ps=ps)
+    assert r["ndiff"] == 0, r
+
+
+SHAPES = [(2, 1, 1, 1), (2, 3, 2, 2), (3, 2, 3, 1), (4, 3, 2, 2), (5, 4, 4, 4), (6, 2, 1, 3),
+          (7, 3, 3, 2), (8, 2, 2, 2), (9, 2, 2, 1), (10, 1, 2, 2), (11, 2, 1, 1), (12, 2, 2, 2),
+          (13, 2, 1, 2), (5, 1, 1, 1), (3, 7, 1, 1)]
+
+
+@pytest.mark.parametrize("kind", [fa.FILL_INT, fa.FILL_RAND, fa.FILL_REF])
+def test_fill_matches_oracle(torch_cuda, kind):
+    shape = (7, 5, 3, 2)
+    want = oracle.new_field(*shape, kind, seed=42)
+    d = torch_cuda.zeros(want.size, dtype=torch_cuda.float64, device="cuda")
+    fa.fill(d, *shape, kind=kind, seed=42)
+    assert np.array_equal(to_host(torch_cuda, d), want)
+
+
+def test_fill_padded(torch_cuda):
+    shape = (5, 3, 2, 3)
+    n, mx, my, mz = shape
+    ps = n ** 3 * mx * my + 512
+    want = oracle.new_field(*shape, fa.FILL_RAND, seed=3, ps=ps)
+    d = torch_cuda.zeros(want.size, dtype=torch_cuda.float64, device="cuda")
+    fa.fill(d, *shape, kind=fa.FILL_RAND, seed=3, plane_stride=ps)
+    assert np.array_equal(to_host(torch_cuda, d), want)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("kind", [fa.FILL_INT, fa.FILL_RAND])
+def test_periodic_fused_bitexact(torch_cuda, shape, kind):
+    print("case periodic fused", shape, kind, flush=True)
+    u = oracle.new_field(*shape, kind, seed=101)
+    assert_same(run_gpu(torch_cuda, u, shape), run_oracle(u, shape), shape)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_open_bitexact(torch_cuda, shape):
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=5)
+    got = run_gpu(torch_cuda, u, shape, boundary=fa.OPEN)
+    assert_same(got, run_oracle(u, shape, periodic=False), shape)
+
+
+@pytest.mark.parametrize("shape", [(5, 4, 4, 4), (7, 3, 3, 2), (2, 2, 3, 1), (12, 2, 1, 2)])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_compute_false(torch_cuda, shape, periodic):
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=8)
+    got = run_gpu(torch_cuda, u, shape, boundary=fa.PERIODIC if periodic else fa.OPEN,
+                  compute=False)
+    assert_same(got, run_oracle(u, shape, periodic=periodic, compute=False), shape)
+
+
+@pytest.mark.parametrize("path", [fa.PATH_FUSED, fa.PATH_STAGED, fa.PATH_STAGED_DIRECT])
+@pytest.mark.parametrize("shape", [(5, 4, 4, 4), (7, 3, 2, 2), (2, 1, 2, 1), (8, 2, 3, 2)])
+@pytest.mark.parametrize("compute", [True, False])
+def test_paths_agree_periodic(torch_cuda, path, shape, compute):
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=13)
+    got = run_gpu(torch_cuda, u, shape, compute=compute, path=path)
+    assert_same(got, run_oracle(u, shape, compute=compute), shape)
+
+
+@pytest.mark.parametrize("tuning", [(64, 0, 0), (128, 1, 1), (512, 2, 1), (1024, 3, 0),
+                                    (256, 0, 1)])
+def test_tuning_does_not_change_bits(torch_cuda, tuning):
+    shape = (7, 5, 3, 4)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=77)
+    got = run_gpu(torch_cuda, u, shape, tuning=tuning)
+    assert_same(got, run_oracle(u, shape), shape)
+
+
+def test_padded_plane_stride(torch_cuda):
+    shape = (7, 3, 2, 3)
+    n, mx, my, mz = shape
+    plane = n ** 3 * mx * my
+    ps = ((plane - 1) // 512 + 1) * 512  # gpu/DArray.h:95-99
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=19, ps=ps)
+    for periodic in (True, False):
+        got = run_gpu(torch_cuda, u, shape, ps=ps,
+                      boundary=fa.PERIODIC if periodic else fa.OPEN)
+        want = run_oracle(u, shape, periodic=periodic, ps=ps)
+        assert np.array_equal(got, want)  # padding slots included
+
+
+def test_pack_messages_match_oracle(torch_cuda):
+    shape = (5, 3, 4, 2)
+    n, mx, my, mz = shape
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=31)
+    msg, off = oracle.pack(u, *shape)
+    d = to_dev(torch_cuda, u)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    f.pack(d)
+    torch_cuda.cuda.synchronize()
+    base, _ = f.message(0)
+    arena_end, cnt_end = f.message(25, recv=True)
+    nbytes = arena_end + 8 * cnt_end - base
+    import ctypes
+    host = np.zeros(nbytes // 8)
+    # copy the whole arena to the host through torch
+    t = torch_cuda.empty(host.size, dtype=torch_cuda.float64, device="cuda")
+    torch_cuda.cuda.synchronize()
+    lib = ctypes.CDLL("libamdhip64.so.7")  # the runtime torch already loaded
+    lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert lib.hipMemcpy(host.ctypes.data, base, nbytes, 2) == 0  # device -> host
+    del t
+    for k in range(26):
+        p, c = f.message(k)
+        o = (p - base) // 8
+        assert c == off[k + 1] - off[k]
+        assert np.array_equal(host[o:o + c], msg[off[k]:off[k + 1]]), k
+    # exchange: recv[k] == send[opposite(k)]
+    f.exchange()
+    torch_cuda.cuda.synchronize()
+    assert lib.hipMemcpy(host.ctypes.data, base, nbytes, 2) == 0
+    for k in range(26):
+        p, c = f.message(k, recv=True)
+        o = (p - base) // 8
+        ok = oracle.opposite_slot(k)
+        assert np.array_equal(host[o:o + c], msg[off[ok]:off[ok + 1]]), k
+    f.close()
+
+
+@pytest.mark.parametrize("lgrid", [(3, 3, 3), (2, 1, 3), (1, 2, 1)])
+def test_rank_positions_open(torch_cuda, lgrid):
+    """One GPU standing in for each rank of an open decomposition: existing
+    neighbours are self halos; missing ones are MPI_PROC_NULL."""
+    lx, ly, lz = lgrid
+    shape = (5, 2, 3, 2)
+    n, mx, my, mz = shape
+    for rank in range(lx * ly * lz):
+        mask = brute.mask_for_rank(rank, lx, ly, lz)
+        u = oracle.new_field(*shape, fa.FILL_RAND, seed=rank + 1)
+        want = u.copy()
+        oracle.share_staged(want, *shape, nbr_mask=mask)
+        for path in (fa.PATH_STAGED, fa.PATH_STAGED_DIRECT):
+            got = run_gpu(torch_cuda, u, shape, boundary=fa.OPEN, rank=rank, lgrid=lgrid,
+                          path=path)
+            assert_same(got, want, shape)
+
+
+def test_neighbor_table(torch_cuda):
+    f = fa.Faces(13, 3, 3, 3, 2, 2, 2, 4, boundary=fa.OPEN)  # centre rank
+    assert all(f.neighbor(k) >= 0 for k in range(26))
+    assert f.neighbor(0) == 12 and f.neighbor(1) == 14  # x-, x+
+    assert f.neighbor(18) == 0 and f.neighbor(25) == 26  # corners
+    f.close()
+    g = fa.Faces(0, 3, 3, 3, 2, 2, 2, 4, boundary=fa.OPEN)
+    assert g.neighbor(0) == -1 and g.neighbor(1) == 1 and g.neighbor(25) == 13
+    g.close()
+
+
+def test_repeated_shares(torch_cuda):
+    shape = (5, 4, 4, 4)  # the CPU-runnable configuration, nk=nj=ni=1 .. 3
+    u = oracle.new_field(*shape, fa.FILL_INT, seed=42)
+    got = run_gpu(torch_cuda, u, shape, nshare=3)
+    assert_same(got, run_oracle(u, shape, nshare=3), shape)
+
+
+def test_c1_configuration(torch_cuda):
+    """1x1x1 periodic, 4^3 elements, n=5, integer [0,1023], seed 42."""
+    shape = (5, 4, 4, 4)
+    u = oracle.new_field(*shape, fa.FILL_INT, seed=42)
+    got = run_gpu(torch_cuda, u, shape)
+    assert_same(got, run_oracle(u, shape), shape)
+
+
+def test_c3_configuration(torch_cuda):
+    """One rank of 6x6x6: 32^3 elements, n=8, random FP64 [-1,1), self halos."""
+    shape = (8, 32, 32, 32)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=42)
+    want = run_oracle(u, shape)
+    got = run_gpu(torch_cuda, u, shape)
+    assert_same(got, want, shape)
+    got2 = run_gpu(torch_cuda, u, shape, rank=0, lgrid=(6, 6, 6))
+    assert_same(got2, want, shape)
+
+
+def test_c2_configuration(torch_cuda):
+    """1x1x1 periodic, 60^3 elements, n=7, integer [0,1023], ni=10 shares."""
+    shape = (7, 60, 60, 60)
+    n, mx, my, mz = shape
+    total = n ** 3 * mx * my * mz
+    d = torch_cuda.empty(total, dtype=torch_cuda.float64, device="cuda")
+    fa.fill(d, *shape, kind=fa.FILL_INT, seed=42)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    for _ in range(10):
+        f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    del d
+    want = oracle.new_field(*shape, fa.FILL_INT, seed=42)
+    for _ in range(10):
+        oracle.share_sweep(want, *shape, periodic=True)
+    assert_same(got, want, shape)
+
+
+def test_invalid_arguments(torch_cuda):
+    with pytest.raises(fa.FacesError) as e:
+        fa.Faces(0, 1, 1, 1, 2, 2, 2, 1)
+    assert e.value.status == fa.ERR_ARG
+    with pytest.raises(fa.FacesError):
+        fa.Faces(5, 2, 2, 1, 2, 2, 2, 4)
+    with pytest.raises(fa.FacesError):
+        fa.Faces(0, 1, 1, 1, 2, 2, 2, 4, plane_stride=10)
+    f = fa.Faces(0, 2, 1, 1, 2, 2, 2, 4, boundary=fa.OPEN)
+    with pytest.raises(fa.FacesError):
+        f.set_path(fa.PATH_FUSED)  # rank with a neighbour needs the staged path
+    with pytest.raises(fa.FacesError):
+        f.set_tuning(100, 0, 0)
+    f.close()
