@@ -46,6 +46,11 @@ def parse():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--seg", type=int, default=-1)
     ap.add_argument("--xcd", type=int, default=-1)
+    ap.add_argument("--variant", type=int, default=0, help="FACES_VARIANT_* (0 auto)")
+    ap.add_argument("--vblock", type=int, default=0,
+                    help="LDS-kernel threads per block, or streaming-kernel blocks per CU")
+    ap.add_argument("--opt", action="append", default=[],
+                    help="plan option NAME=VALUE (faces_amd OPT_* without the prefix)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-kernel reference")
@@ -121,6 +126,11 @@ def main():
     f.set_path(args.path)
     if args.block or args.seg >= 0 or args.xcd >= 0:
         f.set_tuning(args.block or 256, max(args.seg, 0), 1 if args.xcd < 0 else args.xcd)
+    if args.variant or args.vblock:
+        f.set_variant(args.variant, args.vblock)
+    for o in args.opt:
+        name, val = o.split("=")
+        f.set_option(getattr(fa, "OPT_" + name.upper()), int(val))
     stream = torch.cuda.current_stream()
 
     for _ in range(args.warmup):

@@ -31,6 +31,9 @@ HEADER = os.path.join(ROOT, "include", "faces_amd.h")
 
 OPEN, PERIODIC = 0, 1
 PATH_AUTO, PATH_FUSED, PATH_STAGED, PATH_STAGED_DIRECT = 0, 1, 2, 3
+VARIANT_AUTO, VARIANT_DIRECT, VARIANT_LDS, VARIANT_STREAM = 0, 1, 2, 3
+(OPT_VARIANT, OPT_DIRECT_BLOCK, OPT_SEG, OPT_XCD, OPT_LDS_BLOCK, OPT_STREAM_BPC,
+ OPT_STREAM_WALK, OPT_ACTIVE_KERNEL) = range(8)
 FILL_INT, FILL_RAND, FILL_REF = 0, 1, 2
 
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_ADMISSION, ERR_NODEV, ERR_TIMEOUT = (
@@ -77,6 +80,9 @@ _sig("faces_plan_destroy", _c_int, _vp)
 _sig("faces_share", _c_int, _vp, _vp, _c_int, _vp)
 _sig("faces_plan_set_path", _c_int, _vp, _c_int)
 _sig("faces_plan_set_tuning", _c_int, _vp, _c_int, _c_int, _c_int)
+_sig("faces_plan_set_variant", _c_int, _vp, _c_int, _c_int)
+_sig("faces_plan_set_option", _c_int, _vp, _c_int, _c_int)
+_sig("faces_plan_get_option", _c_int, _vp, _c_int, ctypes.POINTER(_c_int))
 _sig("faces_plan_neighbor", _c_int, _vp, _c_int, ctypes.POINTER(_c_int))
 _sig("faces_plan_message", _c_int, _vp, _c_int, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_long))
 _sig("faces_pack", _c_int, _vp, _vp, _vp)
@@ -172,6 +178,17 @@ class Faces:
     def set_tuning(self, block=256, seg_len=0, xcd_remap=1) -> None:
         _check(_lib.faces_plan_set_tuning(self._h, block, seg_len, xcd_remap),
                "faces_plan_set_tuning")
+
+    def set_variant(self, variant: int, block: int = 0) -> None:
+        _check(_lib.faces_plan_set_variant(self._h, variant, block), "faces_plan_set_variant")
+
+    def set_option(self, option: int, value: int) -> None:
+        _check(_lib.faces_plan_set_option(self._h, option, value), "faces_plan_set_option")
+
+    def get_option(self, option: int) -> int:
+        v = _c_int()
+        _check(_lib.faces_plan_get_option(self._h, option, ctypes.byref(v)), "faces_plan_get_option")
+        return v.value
 
     def neighbor(self, slot: int) -> int:
         r = _c_int()

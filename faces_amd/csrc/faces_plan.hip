@@ -8,6 +8,7 @@
 
 #include "faces_common.hpp"
 #include "faces_kernels.hpp"
+#include "faces_stream.hpp"
 
 namespace faces {
 
@@ -58,6 +59,10 @@ struct faces_plan {
   int mask;      // face-neighbour bits x-,x+,y-,y+,z-,z+
   int path;      // FACES_PATH_*
   int block, seg_len, xcd_remap;
+  int variant;   // FACES_VARIANT_*
+  int lds_block; // threads per block of the LDS-staged kernel
+  int stream_bpc; // persistent blocks per CU of the streaming kernel
+  int stream_walk;   // streaming kernel traversal: 0 items, 1 x walk, 2 y walk
   // message arena: send slots then receive slots
   double *arena;
   long arena_doubles;
@@ -157,8 +162,149 @@ int grid_for(long work, int block)
   return (int)g;
 }
 
+// Elements per item of the LDS kernel and the LDS bytes that takes.
+int elem_seg(const faces_plan *p)
+{
+  const int want = p->seg_len > 0 ? p->seg_len : 4;
+  return want > p->mx ? p->mx : want;
+}
+
+long elem_lds_bytes(const faces_plan *p, int S)
+{
+  const long n = p->n, K = n - 1;
+  return (long)sizeof(double) * (S * (K * n * n + 2 * K * n + n) + n * n);
+}
+
+bool elem_lds_ok(const faces_plan *p)
+{
+  return p->n >= 2 && p->n <= 12 && elem_lds_bytes(p, elem_seg(p)) <= 64 * 1024;
+}
+
+int launch_elem_lds(const faces_plan *p, double *u, hipStream_t st)
+{
+  Geom G = make_geom(p, u);
+  ElemItems I;
+  I.my = p->my;
+  I.mz = p->mz;
+  I.seg = elem_seg(p);
+  I.nseg = (p->mx + I.seg - 1) / I.seg;
+  I.nitems = p->my * p->mz * I.nseg;
+  I.xcd_remap = p->xcd_remap;
+  const int nb = I.xcd_remap ? 8 * ((I.nitems + 7) / 8) : I.nitems;
+  const size_t lds = (size_t)elem_lds_bytes(p, I.seg);
+  const dim3 grid(nb), blk(p->lds_block);
+#define FACES_LAUNCH_ELEM(NN) k_elem_lds<NN, 8><<<grid, blk, lds, st>>>(G, I);
+  switch (p->n) {
+    case 2: FACES_LAUNCH_ELEM(2); break;
+    case 3: FACES_LAUNCH_ELEM(3); break;
+    case 4: FACES_LAUNCH_ELEM(4); break;
+    case 5: FACES_LAUNCH_ELEM(5); break;
+    case 6: FACES_LAUNCH_ELEM(6); break;
+    case 7: FACES_LAUNCH_ELEM(7); break;
+    case 8: FACES_LAUNCH_ELEM(8); break;
+    case 9: FACES_LAUNCH_ELEM(9); break;
+    case 10: FACES_LAUNCH_ELEM(10); break;
+    case 11: FACES_LAUNCH_ELEM(11); break;
+    case 12: FACES_LAUNCH_ELEM(12); break;
+    default: return fail(FACES_ERR_ARG, "LDS kernel has no n=%d specialisation", p->n);
+  }
+#undef FACES_LAUNCH_ELEM
+  FACES_HIP(hipGetLastError());
+  return FACES_OK;
+}
+
+// Footprint slots per thread of the streaming kernel for (n, S); mirrors
+// StreamShape<N,S>::J.
+int stream_slots(int n, int S)
+{
+  const int K = n - 1, T = kStreamThreads;
+  const int own = S * K * n * n, part = 2 * S * K * n + S * n + n * n;
+  return (own + T - 1) / T + (part + T - 1) / T;
+}
+
+constexpr int kStreamMaxSlots = 12;
+
+// Elements per item: the largest S in {4,3,2,1} not above the requested
+// length that divides mx and keeps the per-thread tables in registers.
+int stream_seg(const faces_plan *p)
+{
+  if (p->n < 2 || p->n > 12) return 0;
+  const int want = p->seg_len > 0 ? p->seg_len : 4;
+  for (int s = want < 4 ? want : 4; s >= 1; s--) {
+    if (p->mx % s) continue;
+    if (stream_slots(p->n, s) <= kStreamMaxSlots) return s;
+  }
+  return 0;
+}
+
+// Byte offsets of the streaming kernel are 32-bit.
+bool stream_ok(const faces_plan *p)
+{
+  const long bytes = 8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps);
+  return stream_seg(p) > 0 && bytes < 0xFFFFFFF0L;
+}
+
+template <int N, int S>
+int launch_stream_ns(const faces_plan *p, const Geom &G, hipStream_t st)
+{
+  using SH = StreamShape<N, S>;
+  StreamItems I;
+  I.my = p->my;
+  I.mz = p->mz;
+  I.nseg = p->mx / S;
+  I.nitems = p->my * p->mz * I.nseg;
+  I.field_bytes = (unsigned)(8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps));
+  I.xcd_remap = p->xcd_remap;
+  I.walk = p->stream_walk;
+  const long units = (I.walk == 1) ? (long)p->my * p->mz : (I.walk == 2) ? (long)p->mz * I.nseg
+                                                                         : I.nitems;
+  // stream_bpc = 0: one unit per block (no persistence)
+  long nb = p->stream_bpc > 0 ? (long)device_cus() * p->stream_bpc : units;
+  if (nb > units) nb = units;
+  if (I.xcd_remap) nb = (nb + 7) / 8 * 8;
+  k_stream<N, S><<<dim3((unsigned)nb), dim3(kStreamThreads), SH::P * sizeof(double), st>>>(G, I);
+  FACES_HIP(hipGetLastError());
+  return FACES_OK;
+}
+
+template <int N>
+int launch_stream_n(const faces_plan *p, const Geom &G, int S, hipStream_t st)
+{
+  switch (S) {
+    case 1: return launch_stream_ns<N, 1>(p, G, st);
+    case 2: return launch_stream_ns<N, 2>(p, G, st);
+    case 3: return launch_stream_ns<N, 3>(p, G, st);
+    default: return launch_stream_ns<N, 4>(p, G, st);
+  }
+}
+
+int launch_stream(const faces_plan *p, double *u, hipStream_t st)
+{
+  const Geom G = make_geom(p, u);
+  const int S = stream_seg(p);
+  switch (p->n) {
+    case 2: return launch_stream_n<2>(p, G, S, st);
+    case 3: return launch_stream_n<3>(p, G, S, st);
+    case 4: return launch_stream_n<4>(p, G, S, st);
+    case 5: return launch_stream_n<5>(p, G, S, st);
+    case 6: return launch_stream_n<6>(p, G, S, st);
+    case 7: return launch_stream_n<7>(p, G, S, st);
+    case 8: return launch_stream_n<8>(p, G, S, st);
+    case 9: return launch_stream_n<9>(p, G, S, st);
+    case 10: return launch_stream_n<10>(p, G, S, st);
+    case 11: return launch_stream_n<11>(p, G, S, st);
+    case 12: return launch_stream_n<12>(p, G, S, st);
+    default: return fail(FACES_ERR_ARG, "streaming kernel has no n=%d specialisation", p->n);
+  }
+}
+
 int launch_box(const faces_plan *p, double *u, bool skip_surface, hipStream_t st)
 {
+  if (!skip_surface && (p->variant == FACES_VARIANT_AUTO || p->variant == FACES_VARIANT_STREAM) &&
+      stream_ok(p))
+    return launch_stream(p, u, st);
+  if (!skip_surface && p->variant == FACES_VARIANT_LDS && elem_lds_ok(p))
+    return launch_elem_lds(p, u, st);
   Geom G = make_geom(p, u);
   RowItems R;
   R.my = p->my;
@@ -323,6 +469,10 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
   p->block = 256;
   p->seg_len = 0;
   p->xcd_remap = 1;
+  p->variant = FACES_VARIANT_AUTO;
+  p->lds_block = 256;
+  p->stream_bpc = 2;   // measured best on 60^3 n=7 and 32^3 n=8
+  p->stream_walk = 1;
 
   // Message sizes, gpu/v-26-messages/Faces.cc:44-47, 64-66.
   const long nn = (long)n * n;
@@ -395,6 +545,80 @@ int faces_plan_set_tuning(faces_plan *p, int block, int seg_len, int xcd_remap)
   p->block = block;
   p->seg_len = seg_len;
   p->xcd_remap = xcd_remap ? 1 : 0;
+  return FACES_OK;
+}
+
+int faces_plan_set_variant(faces_plan *p, int variant, int block)
+{
+  if (!p) return fail(FACES_ERR_ARG, "null plan");
+  if (variant < FACES_VARIANT_AUTO || variant > FACES_VARIANT_STREAM)
+    return fail(FACES_ERR_ARG, "unknown kernel variant %d", variant);
+  if (variant == FACES_VARIANT_STREAM) {
+    if (block < -1 || block > 32)
+      return fail(FACES_ERR_ARG, "blocks per CU %d outside [-1,32] (-1: one item per block)", block);
+    if (!stream_ok(p))
+      return fail(FACES_ERR_ARG, "streaming kernel does not fit this grid (n=%d, mx=%d)", p->n, p->mx);
+    p->variant = variant;
+    if (block) p->stream_bpc = block < 0 ? 0 : block;
+    return FACES_OK;
+  }
+  if (block != 0 && (block < 64 || block > 512 || block % 64))
+    return fail(FACES_ERR_ARG, "LDS kernel block %d not a multiple of 64 in [64,512]", block);
+  if (variant == FACES_VARIANT_LDS && !elem_lds_ok(p))
+    return fail(FACES_ERR_ARG, "LDS row kernel does not fit this grid (n=%d, mx=%d)", p->n, p->mx);
+  p->variant = variant;
+  if (block) p->lds_block = block;
+  return FACES_OK;
+}
+
+int faces_plan_set_option(faces_plan *p, int option, int value)
+{
+  if (!p) return fail(FACES_ERR_ARG, "null plan");
+  switch (option) {
+    case FACES_OPT_VARIANT: return faces_plan_set_variant(p, value, 0);
+    case FACES_OPT_DIRECT_BLOCK: return faces_plan_set_tuning(p, value, p->seg_len, p->xcd_remap);
+    case FACES_OPT_SEG:
+      if (value < 0) return fail(FACES_ERR_ARG, "negative segment length");
+      p->seg_len = value;
+      return FACES_OK;
+    case FACES_OPT_XCD:
+      p->xcd_remap = value ? 1 : 0;
+      return FACES_OK;
+    case FACES_OPT_LDS_BLOCK:
+      if (value < 64 || value > 512 || value % 64)
+        return fail(FACES_ERR_ARG, "LDS kernel block %d not a multiple of 64 in [64,512]", value);
+      p->lds_block = value;
+      return FACES_OK;
+    case FACES_OPT_STREAM_BPC:
+      if (value < 0 || value > 32) return fail(FACES_ERR_ARG, "blocks per CU %d outside [0,32]", value);
+      p->stream_bpc = value;
+      return FACES_OK;
+    case FACES_OPT_STREAM_WALK:
+      if (value < 0 || value > 2) return fail(FACES_ERR_ARG, "walk %d outside [0,2]", value);
+      p->stream_walk = value;
+      return FACES_OK;
+    default: return fail(FACES_ERR_ARG, "unknown option %d", option);
+  }
+}
+
+int faces_plan_get_option(const faces_plan *p, int option, int *value)
+{
+  if (!p || !value) return fail(FACES_ERR_ARG, "null argument");
+  switch (option) {
+    case FACES_OPT_VARIANT: *value = p->variant; break;
+    case FACES_OPT_DIRECT_BLOCK: *value = p->block; break;
+    case FACES_OPT_SEG: *value = p->seg_len; break;
+    case FACES_OPT_XCD: *value = p->xcd_remap; break;
+    case FACES_OPT_LDS_BLOCK: *value = p->lds_block; break;
+    case FACES_OPT_STREAM_BPC: *value = p->stream_bpc; break;
+    case FACES_OPT_STREAM_WALK: *value = p->stream_walk; break;
+    case FACES_OPT_ACTIVE_KERNEL:
+      *value = (p->variant == FACES_VARIANT_DIRECT) ? FACES_VARIANT_DIRECT
+               : (p->variant == FACES_VARIANT_LDS && elem_lds_ok(p)) ? FACES_VARIANT_LDS
+               : stream_ok(p) ? FACES_VARIANT_STREAM : FACES_VARIANT_DIRECT;
+      break;
+    default: return fail(FACES_ERR_ARG, "unknown option %d", option);
+  }
   return FACES_OK;
 }
 

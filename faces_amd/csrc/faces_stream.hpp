@@ -1,0 +1,370 @@
+// faces_stream.hpp -- the streaming in-place DSS kernel for periodic fields.
+//
+// Work item: S consecutive elements [jx0, jx0+S) of one element row (jy,jz),
+// in memory order.  The item owns the lower-corner nodes (ix,iy,iz < n-1) of
+// its elements; their copies form the item's footprint:
+//   OWN  its elements, offsets [0, (n-1)*n^2) of each (planes iz < n-1; the
+//        x-rows iy = n-1 in there are copies of the next row's nodes: loaded,
+//        never stored)
+//   Y    row (jy-1,jz):   x-rows iy = n-1 of planes iz < n-1
+//   Z    row (jy,jz-1):   plane iz = n-1, x-rows iy < n-1
+//   W    row (jy-1,jz-1): plane iz = n-1, x-row iy = n-1
+//   SEAM the ix = n-1 column of element jx0-1 (m-1 when jx0 = 0) in the four
+//        rows: the x partners of the first element's ix = 0 nodes
+// and the ix = n-1 positions of the last element belong to the next item.
+// Every field location is a copy of exactly one node, so each is loaded and
+// stored by exactly one item: the kernel moves the field once in, once out.
+//
+// Cost model.  The footprint layout depends only on (n, S), both template
+// parameters; an item changes nothing but four row bases.  Each thread owns
+// fixed slots t = j*256 + tid of the footprint: OWN fills the first slots
+// (a block-uniform base plus a per-thread offset computed once per launch),
+// Y/Z/W/SEAM the rest (per-lane base picked once per item).  Loads and stores
+// are therefore straight-line code with no per-access index arithmetic, and
+// the compiler can wait for exactly the loads it needs (vmcnt on this
+// architecture also counts the preceding item's stores).  The summation
+// touches only the (n-1)^3 - (n-2)^3 shared nodes of each element, whose LDS
+// positions are precomputed.  Blocks are persistent and take items in
+// XCD-contiguous order; each block issues an item's stores and then, at
+// once, the next item's loads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "faces_kernels.hpp"
+
+namespace faces {
+
+constexpr int kStreamThreads = 256;
+
+template <int N, int S>
+struct StreamShape {
+  static constexpr int n = N, K = N - 1, n2 = N * N, n3 = N * N * N;
+  static constexpr int cOWN = S * K * n2;
+  static constexpr int JO = (cOWN + kStreamThreads - 1) / kStreamThreads;  // OWN slots
+  static constexpr int cY = S * K * n, cZ = S * K * n, cW = S * n, cS = n2;
+  static constexpr int cPART = cY + cZ + cW + cS;
+  static constexpr int JP = (cPART + kStreamThreads - 1) / kStreamThreads;  // other slots
+  static constexpr int J = JO + JP;
+  static constexpr int P = J * kStreamThreads;  // LDS doubles
+  static constexpr int rY = JO * kStreamThreads, rZ = rY + cY, rW = rZ + cZ, rS = rW + cW;
+  static constexpr int nshared = K * K * K - (K - 1) * (K - 1) * (K - 1);
+  static constexpr int NN = (S * nshared + kStreamThreads - 1) / kStreamThreads;
+};
+
+struct StreamItems {
+  int my, mz;
+  int nseg;             // mx / S
+  int nitems;           // my*mz*nseg
+  unsigned field_bytes; // below 2^32 - 8: offsets are 32-bit
+  int xcd_remap;        // 1: contiguous ranges per XCD
+  int walk;             // 0 items, 1 rows in x (seam carried), 2 columns in y (y rows carried)
+};
+
+template <int N, int S>
+__global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I)
+{
+  using SH = StreamShape<N, S>;
+  constexpr int n = N, K = N - 1, n2 = N * N;
+  constexpr int J = SH::J, JO = SH::JO, NN = SH::NN;
+  constexpr long n3 = (long)N * N * N;
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x;
+
+  // ---- per-thread footprint table (item-invariant)
+  unsigned voff[J];
+  unsigned keepm = 0;
+  unsigned rowsel[SH::JP];  // PART slots: row 0 own, 1 (jy-1), 2 (jz-1), 3 both; bit 2 = seam
+  // Walks carry part of the footprint from the previous item's LDS instead of
+  // reloading it: the seam column (x walk) or the y-partner rows (y walk).
+  unsigned carrym = 0;         // PART slots whose value is carried
+  unsigned carrysrc[SH::JP];   // their LDS position in the previous item
+#pragma unroll
+  for (int j = 0; j < JO; j++) {
+    const int q = j * kStreamThreads + tid;
+    voff[j] = 0;
+    if (q < SH::cOWN) {
+      const int e = q / (K * n2), rr = q - e * (K * n2);
+      const int ix = rr % n, iy = (rr / n) % n;
+      voff[j] = e * n3 + rr;
+      if (iy < K && !(e == S - 1 && ix == K)) keepm |= 1u << j;
+    }
+  }
+#pragma unroll
+  for (int jp = 0; jp < SH::JP; jp++) {
+    const int j = JO + jp;
+    int q = jp * kStreamThreads + tid;
+    unsigned off = 0, rs = 0;
+    bool keep = false;
+    if (q < SH::cY) {
+      const int e = q / (K * n), rr = q - e * (K * n);
+      const int iz = rr / n, ix = rr - (rr / n) * n;
+      off = e * n3 + iz * n2 + K * n + ix;
+      rs = 1;
+      keep = !(e == S - 1 && ix == K);
+    } else if ((q -= SH::cY) < SH::cZ) {
+      const int e = q / (K * n), rr = q - e * (K * n);
+      const int ix = rr % n;
+      off = e * n3 + K * n2 + rr;
+      rs = 2;
+      keep = !(e == S - 1 && ix == K);
+    } else if ((q -= SH::cZ) < SH::cW) {
+      const int e = q / n, ix = q - (q / n) * n;
+      off = e * n3 + K * n2 + K * n + ix;
+      rs = 3;
+      keep = !(e == S - 1 && ix == K);
+    } else if ((q -= SH::cW) < SH::cS) {
+      keep = true;
+      if (q < K * K) {
+        off = (q / K) * n2 + (q % K) * n + K;
+        rs = 4 | 0;
+      } else if (q < K * K + K) {
+        off = (q - K * K) * n2 + K * n + K;
+        rs = 4 | 1;
+      } else if (q < K * K + 2 * K) {
+        off = K * n2 + (q - K * K - K) * n + K;
+        rs = 4 | 2;
+      } else {
+        off = K * n2 + K * n + K;
+        rs = 4 | 3;
+      }
+    }
+    voff[j] = off;
+    rowsel[jp] = rs;
+    if (keep) keepm |= 1u << j;
+    carrysrc[jp] = 0;
+    if (I.walk == 1 && (rs & 4)) {
+      // x walk: the seam is the previous segment's last element's ix = n-1
+      // column, which that item never changes.
+      carrym |= 1u << jp;
+      if (q < K * K) carrysrc[jp] = (S - 1) * (K * n2) + (q / K) * n2 + (q % K) * n + K;
+      else if (q < K * K + K) carrysrc[jp] = SH::rY + (S - 1) * (K * n) + (q - K * K) * n + K;
+      else if (q < K * K + 2 * K) carrysrc[jp] = SH::rZ + (S - 1) * (K * n) + (q - K * K - K) * n + K;
+      else carrysrc[jp] = SH::rW + (S - 1) * n + K;
+    } else if (I.walk == 2 && rs == 1) {
+      // y walk: the y-partner rows are the previous row's x-rows iy = n-1,
+      // loaded (never changed) by that item as part of its OWN region.
+      const int qy = jp * kStreamThreads + tid;
+      const int e = qy / (K * n), rr = qy - e * (K * n);
+      const int iz = rr / n, ix = rr - (rr / n) * n;
+      carrym |= 1u << jp;
+      carrysrc[jp] = e * (K * n2) + iz * n2 + K * n + ix;
+    }
+  }
+
+  // ---- per-thread shared-node table: 8 LDS positions (16 bits each)
+  unsigned np[NN][4];
+  unsigned nflag[NN];  // bit0 cx, bit1 cy, bit2 cz, bit3 active
+#pragma unroll
+  for (int i = 0; i < NN; i++) {
+    const int k = i * kStreamThreads + tid;
+    nflag[i] = 0;
+    np[i][0] = np[i][1] = np[i][2] = np[i][3] = 0;
+    if (k >= S * SH::nshared) continue;
+    const int e = k / SH::nshared;
+    int r = k - e * SH::nshared;
+    int ix, iy, iz;
+    if (r < K * K) {
+      iz = 0; iy = r / K; ix = r - (r / K) * K;
+    } else if (r < K * K + K * (K - 1)) {
+      r -= K * K;
+      iz = 1 + r / K; iy = 0; ix = r - (r / K) * K;
+    } else {
+      r -= K * K + K * (K - 1);
+      constexpr int K1 = (K > 1) ? K - 1 : 1;  // this branch needs K > 1
+      iz = 1 + r / K1; iy = 1 + r - (r / K1) * K1; ix = 0;
+    }
+    unsigned p[8];
+    p[0] = e * (K * n2) + iz * n2 + iy * n + ix;  // own
+    p[2] = SH::rY + e * (K * n) + iz * n + ix;    // y partner
+    p[4] = SH::rZ + e * (K * n) + iy * n + ix;    // z partner
+    p[6] = SH::rW + e * n + ix;                   // yz partner
+    if (e > 0) {
+      p[1] = (e - 1) * (K * n2) + iz * n2 + iy * n + K;
+      p[3] = SH::rY + (e - 1) * (K * n) + iz * n + K;
+      p[5] = SH::rZ + (e - 1) * (K * n) + iy * n + K;
+      p[7] = SH::rW + (e - 1) * n + K;
+    } else {
+      p[1] = SH::rS + iz * K + iy;
+      p[3] = SH::rS + K * K + iz;
+      p[5] = SH::rS + K * K + K + iy;
+      p[7] = SH::rS + K * K + 2 * K;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; w++) np[i][w] = p[2 * w] | (p[2 * w + 1] << 16);
+    nflag[i] = (ix == 0 ? 1u : 0u) | (iy == 0 ? 2u : 0u) | (iz == 0 ? 4u : 0u) | 8u;
+  }
+
+  // ---- traversal.  Work units are dealt to blocks in XCD-contiguous ranges
+  // (block b runs on XCD b % 8) or in one global order.  A unit is one item
+  // (walk 0), an element row walked in x (walk 1), or a column of rows at
+  // fixed (jz, segment) walked in y (walk 2).  Items that share memory
+  // sectors then run back to back on one CU, whose L2 merges their writes.
+  const int units = (I.walk == 1) ? I.my * I.mz : (I.walk == 2) ? I.mz * I.nseg : I.nitems;
+  int start, cnt, k, nbx;
+  if (I.xcd_remap) {
+    const int xcd = blockIdx.x & 7;
+    nbx = (gridDim.x - xcd + 7) >> 3;  // blocks on this XCD
+    const int q8 = units >> 3, rem8 = units & 7;
+    start = xcd * q8 + (xcd < rem8 ? xcd : rem8);
+    cnt = q8 + (xcd < rem8 ? 1 : 0);
+    k = blockIdx.x >> 3;
+  } else {
+    start = 0;
+    cnt = units;
+    k = blockIdx.x;
+    nbx = gridDim.x;
+  }
+  if (k >= cnt) return;
+  const int nsegu = (I.walk == 1) ? I.nseg : (I.walk == 2) ? G.my : 1;  // items per unit
+  int sg = 0;
+  auto item_of = [&](int kk, int ss) {
+    const int c = start + kk;
+    if (I.walk == 1) return c * I.nseg + ss;
+    if (I.walk == 2) return ((c / I.nseg) * G.my + ss) * I.nseg + (c % I.nseg);
+    return c;
+  };
+
+  // Byte offsets into the field through one buffer resource: a store whose
+  // offset is out of range is dropped by the hardware, so the stores of
+  // positions this item does not own need no branch, every memory
+  // instruction is unconditional, and the compiler waits on exact counts.
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(G.u, 0, (int)I.field_bytes, 0x00020000);
+  constexpr unsigned kOOB = 0xFFFFFFF8u;  // any offset at or above field_bytes is dropped
+#pragma unroll
+  for (int j = 0; j < J; j++) voff[j] *= 8;  // bytes from here on
+  unsigned dropm[J];
+#pragma unroll
+  for (int j = 0; j < J; j++) dropm[j] = (keepm & (1u << j)) ? 0u : kOOB;
+
+  unsigned obase = 0;          // item's OWN base (bytes, block-uniform)
+  unsigned pbase[SH::JP];      // item's per-lane PART bases (bytes)
+  auto bases = [&](int it) {
+    const int sg = it % I.nseg;
+    const int row = it / I.nseg;
+    const int jy = row % G.my, jz = row / G.my;
+    const int jym = (jy == 0) ? G.my - 1 : jy - 1;
+    const int jzm = (jz == 0) ? G.mz - 1 : jz - 1;
+    const int jx0 = sg * S;
+    const int jxs = (jx0 == 0) ? G.mx - 1 : jx0 - 1;
+    const unsigned r0 = (unsigned)((jz * G.ps + jy * G.ys) * 8);
+    const unsigned r1 = (unsigned)((jz * G.ps + jym * G.ys) * 8);
+    const unsigned r2 = (unsigned)((jzm * G.ps + jy * G.ys) * 8);
+    const unsigned r3 = (unsigned)((jzm * G.ps + jym * G.ys) * 8);
+    obase = r0 + (unsigned)(jx0 * n3 * 8);
+#pragma unroll
+    for (int jp = 0; jp < SH::JP; jp++) {
+      const unsigned rs = rowsel[jp];
+      const unsigned rb = ((rs & 3) == 0) ? r0 : ((rs & 3) == 1) ? r1 : ((rs & 3) == 2) ? r2 : r3;
+      pbase[jp] = rb + (unsigned)(((rs & 4) ? jxs : jx0) * n3 * 8);
+    }
+  };
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  double v[J];
+  // carry: a carried slot comes from the previous item's LDS, so its load is
+  // sent out of range (no memory access, the instruction count is unchanged)
+  auto load_item = [&](bool carry) {
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+      unsigned off = ((j < JO) ? obase : pbase[j - JO < 0 ? 0 : j - JO]) + voff[j];
+      if (j >= JO && carry && (carrym & (1u << (j - JO)))) off = kOOB;
+      v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+    }
+  };
+
+  int it = item_of(k, 0);
+  bases(it);
+  load_item(false);
+  bool carry = false;
+  double vcar[SH::JP];
+#pragma unroll
+  for (int jp = 0; jp < SH::JP; jp++) vcar[jp] = 0.0;
+  // J dropped stores (offset out of range): the loop is entered with the same
+  // pattern of outstanding operations as every later iteration -- J loads,
+  // then J stores -- so the waits at the loop head are exact.
+#pragma unroll
+  for (int j = 0; j < J; j++)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, 0.0), rsrc, kOOB - 8 * j, 0, 0);
+  while (true) {
+    __syncthreads();  // the previous item's stores have read LDS
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+      double x = v[j];
+      if (j >= JO && carry && (carrym & (1u << (j - JO)))) x = vcar[j - JO < 0 ? 0 : j - JO];
+      lds[j * kStreamThreads + tid] = x;
+    }
+    // this item's store offsets, then the next item's loads (in flight
+    // during this item's sums and stores)
+    unsigned soff[J];
+#pragma unroll
+    for (int j = 0; j < J; j++)
+      soff[j] = (((j < JO) ? obase : pbase[j - JO < 0 ? 0 : j - JO]) + voff[j]) | dropm[j];
+    int nk = k, ns = sg + 1;
+    if (ns == nsegu) {
+      ns = 0;
+      nk = k + nbx;
+    }
+    const bool more = nk < cnt;
+    const bool ncarry = more && ns > 0;
+    // Unconditional loads keep the wait accounting exact; after the last item
+    // they re-read its own locations (in bounds, never used).
+    const int nit = more ? item_of(nk, ns) : it;
+    bases(nit);
+    load_item(ncarry);
+    __syncthreads();
+    if (ncarry) {
+#pragma unroll
+      for (int jp = 0; jp < SH::JP; jp++)
+        if (carrym & (1u << jp)) vcar[jp] = lds[carrysrc[jp]];
+    }
+    // shared nodes: x pairs, then y, then z
+#pragma unroll
+    for (int i = 0; i < NN; i++) {
+      const unsigned f = nflag[i];
+      if (!(f & 8u)) continue;
+      const bool cx = f & 1u, cy = f & 2u, cz = f & 4u;
+      const unsigned p0 = np[i][0] & 0xffff, p1 = np[i][0] >> 16;
+      const unsigned p2 = np[i][1] & 0xffff, p3 = np[i][1] >> 16;
+      const unsigned p4 = np[i][2] & 0xffff, p5 = np[i][2] >> 16;
+      const unsigned p6 = np[i][3] & 0xffff, p7 = np[i][3] >> 16;
+      const double a0 = cx ? lds[p0] + lds[p1] : lds[p0];
+      double q0 = a0;
+      if (cy) q0 = a0 + (cx ? lds[p2] + lds[p3] : lds[p2]);
+      double res = q0;
+      if (cz) {
+        const double b0 = cx ? lds[p4] + lds[p5] : lds[p4];
+        double q1 = b0;
+        if (cy) q1 = b0 + (cx ? lds[p6] + lds[p7] : lds[p6]);
+        res = q0 + q1;
+      }
+      lds[p0] = res;
+      if (cx) lds[p1] = res;
+      if (cy) {
+        lds[p2] = res;
+        if (cx) lds[p3] = res;
+      }
+      if (cz) {
+        lds[p4] = res;
+        if (cx) lds[p5] = res;
+        if (cy) {
+          lds[p6] = res;
+          if (cx) lds[p7] = res;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < J; j++)
+      __builtin_amdgcn_raw_buffer_store_b64(
+          __builtin_bit_cast(u32x2, lds[j * kStreamThreads + tid]), rsrc, soff[j], 0, 0);
+    if (!more) break;
+    k = nk;
+    sg = ns;
+    it = nit;
+    carry = ncarry;
+  }
+}
+
+}  // namespace faces

@@ -92,6 +92,44 @@ int faces_plan_set_path(faces_plan *plan, int path);
  * at build time). */
 int faces_plan_set_tuning(faces_plan *plan, int block, int seg_len, int xcd_remap);
 
+/* Kernel of the in-place sum on periodic fields: FACES_VARIANT_DIRECT (one
+ * lane per node, copies read straight from HBM), FACES_VARIANT_LDS (element
+ * segments staged through LDS), FACES_VARIANT_STREAM (persistent streaming
+ * kernel with item-invariant offset tables); AUTO picks STREAM when the
+ * grid fits it.  block: threads per block of the LDS kernel (64..512,
+ * multiple of 64), or persistent blocks per CU of the STREAM kernel (1..32,
+ * or -1 for one work item per block); 0 keeps the current value. */
+#define FACES_VARIANT_AUTO 0
+#define FACES_VARIANT_DIRECT 1
+#define FACES_VARIANT_LDS 2
+#define FACES_VARIANT_STREAM 3
+int faces_plan_set_variant(faces_plan *plan, int variant, int block);
+
+/* Launch options of the in-place sum (no counterpart in the reference,
+ * whose launch shape is fixed at build time, gpu/gpu.h:73-78):
+ *   FACES_OPT_VARIANT        FACES_VARIANT_*
+ *   FACES_OPT_DIRECT_BLOCK   threads per block of the DIRECT kernel
+ *   FACES_OPT_SEG            elements per work item (0 = default)
+ *   FACES_OPT_XCD            1 = contiguous work ranges per XCD
+ *   FACES_OPT_LDS_BLOCK      threads per block of the LDS kernel
+ *   FACES_OPT_STREAM_BPC     STREAM kernel: persistent blocks per CU (0 = one
+ *                            work unit per block)
+ *   FACES_OPT_STREAM_WALK    STREAM kernel traversal: 0 = independent items,
+ *                            1 = a block walks element rows in x carrying the
+ *                            seam column, 2 = a block walks columns of rows in
+ *                            y carrying the y-partner rows
+ *   FACES_OPT_ACTIVE_KERNEL  (get only) the variant faces_share launches */
+#define FACES_OPT_VARIANT 0
+#define FACES_OPT_DIRECT_BLOCK 1
+#define FACES_OPT_SEG 2
+#define FACES_OPT_XCD 3
+#define FACES_OPT_LDS_BLOCK 4
+#define FACES_OPT_STREAM_BPC 5
+#define FACES_OPT_STREAM_WALK 6
+#define FACES_OPT_ACTIVE_KERNEL 7
+int faces_plan_set_option(faces_plan *plan, int option, int value);
+int faces_plan_get_option(const faces_plan *plan, int option, int *value);
+
 /* Neighbour rank of message slot 0..25, or -1 for none (MPI_PROC_NULL) --
  * Faces::neighbor, gpu/v-26-messages/Faces.cc:92-103. */
 int faces_plan_neighbor(const faces_plan *plan, int slot, int *rank);

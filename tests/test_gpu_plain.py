@@ -25,13 +25,15 @@ def to_host(torch, d) -> np.ndarray:
 
 
 def run_gpu(torch, u, shape, boundary=fa.PERIODIC, compute=True, path=fa.PATH_AUTO, ps=0,
-            rank=0, lgrid=(1, 1, 1), nshare=1, tuning=None):
+            rank=0, lgrid=(1, 1, 1), nshare=1, tuning=None, variant=None):
     n, mx, my, mz = shape
     d = to_dev(torch, u)
     f = fa.Faces(rank, *lgrid, mx, my, mz, n, boundary=boundary, plane_stride=ps)
     f.set_path(path)
     if tuning:
         f.set_tuning(*tuning)
+    if variant:
+        f.set_variant(*variant)
     for _ in range(nshare):
         f.share(d, compute=compute)
     out = to_host(torch, d)
@@ -83,6 +85,49 @@ def test_periodic_fused_bitexact(torch_cuda, shape, kind):
     assert_same(run_gpu(torch_cuda, u, shape), run_oracle(u, shape), shape)
 
 
+@pytest.mark.parametrize("shape", SHAPES[:-3] + [(5, 1, 1, 1), (3, 7, 1, 1), (4, 1, 5, 1),
+                                                  (4, 1, 1, 6), (6, 3, 1, 1), (2, 1, 1, 2)])
+@pytest.mark.parametrize("variant", [(fa.VARIANT_DIRECT, 0), (fa.VARIANT_LDS, 64),
+                                     (fa.VARIANT_LDS, 256), (fa.VARIANT_LDS, 512),
+                                     (fa.VARIANT_STREAM, 1), (fa.VARIANT_STREAM, 8),
+                                     (fa.VARIANT_STREAM, -1)])
+def test_periodic_variants_bitexact(torch_cuda, shape, variant):
+    print("case periodic variant", shape, variant, flush=True)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=202)
+    assert_same(run_gpu(torch_cuda, u, shape, variant=variant), run_oracle(u, shape), shape)
+
+
+@pytest.mark.parametrize("walk", [0, 1, 2])
+@pytest.mark.parametrize("seg,bpc,xcd", [(1, 0, 1), (2, 1, 0), (3, 2, 1), (4, 3, 1)])
+@pytest.mark.parametrize("shape", [(7, 12, 5, 3), (8, 4, 3, 5), (5, 6, 1, 2), (4, 3, 4, 1),
+                                   (6, 2, 2, 2)])
+def test_stream_walks_bitexact(torch_cuda, walk, seg, bpc, xcd, shape):
+    n, mx, my, mz = shape
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=303)
+    d = to_dev(torch_cuda, u)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+    f.set_option(fa.OPT_STREAM_WALK, walk)
+    f.set_option(fa.OPT_SEG, seg)
+    f.set_option(fa.OPT_STREAM_BPC, bpc)
+    f.set_option(fa.OPT_XCD, xcd)
+    assert f.get_option(fa.OPT_ACTIVE_KERNEL) == fa.VARIANT_STREAM
+    for _ in range(2):
+        f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    assert_same(got, run_oracle(u, shape, nshare=2), shape)
+
+
+def test_periodic_lds_padded_and_repeated(torch_cuda):
+    shape = (7, 6, 5, 4)
+    n, mx, my, mz = shape
+    ps = n ** 3 * mx * my + 1000
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=9, ps=ps)
+    got = run_gpu(torch_cuda, u, shape, ps=ps, nshare=3, variant=(fa.VARIANT_LDS, 256))
+    assert np.array_equal(got, run_oracle(u, shape, ps=ps, nshare=3))
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 def test_open_bitexact(torch_cuda, shape):
     u = oracle.new_field(*shape, fa.FILL_RAND, seed=5)
@@ -109,7 +154,7 @@ def test_paths_agree_periodic(torch_cuda, path, shape, compute):
 
 
 @pytest.mark.parametrize("tuning", [(64, 0, 0), (128, 1, 1), (512, 2, 1), (1024, 3, 0),
-                                    (256, 0, 1)])
+                                    (256, 0, 1), (256, 6, 1), (256, 12, 1)])
 def test_tuning_does_not_change_bits(torch_cuda, tuning):
     shape = (7, 5, 3, 4)
     u = oracle.new_field(*shape, fa.FILL_RAND, seed=77)
