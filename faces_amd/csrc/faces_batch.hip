@@ -1,0 +1,449 @@
+// faces_batch.hip -- the batched step: many independent periodic grids (each
+// its own n, mx, my, mz) summed by one persistent launch over a work list of
+// element-segment items, replayed from a captured HIP graph.  No counterpart
+// in the reference, which steps one grid per process (gpu/main.cc:170-186).
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "faces_batch.hpp"
+#include "faces_common.hpp"
+
+namespace faces {
+
+// ---------------------------------------------------------------------------
+// the batched kernel
+// ---------------------------------------------------------------------------
+// Blocks take items from a shared counter; the item's grid is found by binary
+// search over the grids' first-item offsets.  The block that finishes last
+// resets the counter, so consecutive launches (and graph replays) need no
+// memset.  Counters are updated with vector atomics.
+
+#define FACES_BATCH_CASES(X)                                                              \
+  X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(3, 1) X(3, 2) X(3, 3) X(3, 4) X(4, 1) X(4, 2) X(4, 3) \
+  X(4, 4) X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(6, 1) X(6, 2) X(6, 3) X(6, 4) X(7, 1) X(7, 2) \
+  X(7, 3) X(7, 4) X(8, 1) X(8, 2) X(8, 3) X(8, 4) X(9, 1) X(9, 2) X(9, 3) X(10, 1)        \
+  X(10, 2) X(11, 1) X(11, 2) X(12, 1)
+
+// One out-of-line routine per (n, S): inlining all of them into the kernel's
+// switch makes the register allocator size every path for the union of their
+// live ranges (256 VGPRs and spills); as calls, each keeps its own allocation.
+template <int N, int S>
+__device__ __noinline__ void batch_item(const BatchGrid G, int it, double *lds)
+{
+  stream_item<N, S>(G, it, lds);
+}
+
+__global__ __launch_bounds__(kStreamThreads) void k_batch(BatchLaunch D)
+{
+  extern __shared__ double lds[];
+  __shared__ int s_unit;
+  while (true) {
+    if (threadIdx.x == 0) s_unit = (int)atomicAdd(&D.counter[0], 1u);
+    __syncthreads();
+    const int u = s_unit;
+    __syncthreads();  // all threads have read s_unit before it is overwritten
+    if (u >= D.nunits) break;
+    int lo = 0, hi = D.ngrids - 1;  // last grid with first <= u
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (D.grids[mid].first <= u) lo = mid; else hi = mid - 1;
+    }
+    const BatchGrid G = D.grids[lo];
+    const int it = u - G.first;
+    switch (G.n * 8 + G.S) {
+#define FACES_BATCH_CASE(NN, SS) \
+  case NN * 8 + SS: batch_item<NN, SS>(G, it, lds); break;
+      FACES_BATCH_CASES(FACES_BATCH_CASE)
+#undef FACES_BATCH_CASE
+      default: break;  // excluded on the host
+    }
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&D.counter[1], 1u) == gridDim.x - 1) {
+      atomicExch(&D.counter[0], 0u);
+      atomicExch(&D.counter[1], 0u);
+    }
+  }
+}
+
+bool batch_case_ok(int n, int S)
+{
+  switch (n * 8 + S) {
+#define FACES_BATCH_CASE(NN, SS) \
+  case NN * 8 + SS: return true;
+    FACES_BATCH_CASES(FACES_BATCH_CASE)
+#undef FACES_BATCH_CASE
+    default: return false;
+  }
+}
+
+int batch_lds_bytes(int n, int S)
+{
+  const int T = kStreamThreads, K = n - 1;
+  const int own = S * K * n * n, part = 2 * S * K * n + S * n + n * n;
+  return ((own + T - 1) / T + (part + T - 1) / T) * T * (int)sizeof(double);
+}
+
+int batch_seg(int n, int mx, int cap)
+{
+  for (int s = std::min(cap, 4); s >= 1; s--)
+    if (mx % s == 0 && batch_case_ok(n, s) && batch_lds_bytes(n, s) <= 64 * 1024) return s;
+  return 0;
+}
+
+int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t st)
+{
+  if (D.nunits == 0) return FACES_OK;
+  k_batch<<<dim3(nblocks), dim3(kStreamThreads), lds_bytes, st>>>(D);
+  FACES_HIP(hipGetLastError());
+  return FACES_OK;
+}
+
+int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits)
+{
+  // Items of expensive grids first: the work list then ends on cheap items.
+  std::vector<int> order(g.size());
+  std::iota(order.begin(), order.end(), 0);
+  for (auto &x : g) {
+    x.S = batch_seg(x.n, x.mx, cap);
+    if (x.S == 0) return fail(FACES_ERR_ARG, "grid n=%d mx=%d has no batched item shape", x.n, x.mx);
+    x.nseg = x.mx / x.S;
+    x.items = x.my * x.mz * x.nseg;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (long)g[a].S * g[a].n * g[a].n * g[a].n > (long)g[b].S * g[b].n * g[b].n * g[b].n;
+  });
+  std::vector<BatchGrid> sorted;
+  int first = 0, lds = 0;
+  for (int i : order) {
+    BatchGrid x = g[i];
+    x.first = first;
+    first += x.items;
+    lds = std::max(lds, batch_lds_bytes(x.n, x.S));
+    sorted.push_back(x);
+  }
+  g.swap(sorted);
+  *lds_bytes = lds;
+  *nunits = first;
+  return FACES_OK;
+}
+
+}  // namespace faces
+
+using namespace faces;
+
+struct faces_batch {
+  int ngrids;
+  int boundary;
+  std::vector<int> n, mx, my, mz;
+  std::vector<double *> u;
+  std::vector<faces_plan *> plans;  // one-launch-per-grid baseline and fallback paths
+  std::vector<int> solo;            // grids without a batched item shape (n > 12): own plan
+  // device state
+  BatchGrid *d_grids = nullptr;
+  unsigned *d_counter = nullptr;
+  int lds_bytes = 0, nunits = 0, nbatched = 0;
+  // tuning
+  int seg_cap = 4, bpc = 2;
+  // captured graph of the batched launch
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool dirty = true;  // descriptors or tuning changed since the last upload
+};
+
+namespace {
+
+bool batchable(int n) { return n >= 2 && n <= 12; }
+
+int upload(faces_batch *b)
+{
+  std::vector<BatchGrid> g;
+  b->solo.clear();
+  for (int i = 0; i < b->ngrids; i++) {
+    if (!batchable(b->n[i])) {
+      b->solo.push_back(i);
+      continue;
+    }
+    g.emplace_back();
+    BatchGrid &x = g.back();
+    x.u = b->u[i];
+    x.n = b->n[i];
+    x.mx = b->mx[i];
+    x.my = b->my[i];
+    x.mz = b->mz[i];
+    x.ys = (long)x.n * x.n * x.n * x.mx;
+    x.ps = x.ys * x.my;
+    x.field_bytes = (unsigned)(8 * x.ps * x.mz);
+  }
+  b->lds_bytes = 0;
+  b->nunits = 0;
+  if (!g.empty()) {
+    int st = plan_grids(g, b->seg_cap, &b->lds_bytes, &b->nunits);
+    if (st) return st;
+    // launches of the old descriptors may still run on any stream
+    FACES_HIP(hipDeviceSynchronize());
+    FACES_HIP(hipMemcpy(b->d_grids, g.data(), sizeof(BatchGrid) * g.size(), hipMemcpyHostToDevice));
+  }
+  b->nbatched = (int)g.size();
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  b->dirty = false;
+  return FACES_OK;
+}
+
+BatchLaunch launch_desc(const faces_batch *b)
+{
+  BatchLaunch D;
+  D.grids = b->d_grids;
+  D.ngrids = b->nbatched;
+  D.nunits = b->nunits;
+  D.counter = b->d_counter;
+  return D;
+}
+
+int nblocks_of(const faces_batch *b)
+{
+  long nb = (long)device_cus() * b->bpc;
+  if (nb > b->nunits) nb = b->nunits;
+  return (int)std::max(1L, nb);
+}
+
+int ensure_graph(faces_batch *b)
+{
+  if (b->dirty) {
+    int st = upload(b);
+    if (st) return st;
+  }
+  if (b->exec || b->nunits == 0) return FACES_OK;
+  FACES_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+  int st = launch_batch(launch_desc(b), nblocks_of(b), b->lds_bytes, b->cap_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+  if (st) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  b->graph = g;
+  FACES_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return FACES_OK;
+}
+
+bool fields_bound(const faces_batch *b)
+{
+  for (double *p : b->u)
+    if (!p) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int faces_batch_create(faces_batch **out, int ngrids, const int *n, const int *mx, const int *my,
+                       const int *mz, int boundary)
+{
+  if (!out) return fail(FACES_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (ngrids < 1) return fail(FACES_ERR_ARG, "need at least one grid");
+  if (!n || !mx || !my || !mz) return fail(FACES_ERR_ARG, "null shape array");
+  if (boundary != FACES_BOUNDARY_OPEN && boundary != FACES_BOUNDARY_PERIODIC)
+    return fail(FACES_ERR_ARG, "unknown boundary %d", boundary);
+  faces_batch *b = new (std::nothrow) faces_batch();
+  if (!b) return fail(FACES_ERR_NOMEM, "batch allocation");
+  b->ngrids = ngrids;
+  b->boundary = boundary;
+  long total_units = 0;
+  for (int i = 0; i < ngrids; i++) {
+    if (n[i] < 2 || n[i] > 64 || mx[i] < 1 || my[i] < 1 || mz[i] < 1) {
+      delete b;
+      return fail(FACES_ERR_ARG, "grid %d: shape n=%d m=%dx%dx%d invalid", i, n[i], mx[i], my[i], mz[i]);
+    }
+    const long bytes = 8 * faces_field_doubles(n[i], mx[i], my[i], mz[i], 0);
+    if (batchable(n[i]) && bytes >= 0xFFFFFFF0L) {
+      delete b;
+      return fail(FACES_ERR_ARG, "grid %d: field of %ld bytes exceeds the batched step's 4 GB limit", i,
+                  bytes);
+    }
+    total_units += (long)mx[i] * my[i] * mz[i];
+    b->n.push_back(n[i]);
+    b->mx.push_back(mx[i]);
+    b->my.push_back(my[i]);
+    b->mz.push_back(mz[i]);
+  }
+  if (total_units >= (1L << 30)) {
+    delete b;
+    return fail(FACES_ERR_ARG, "batch too large");
+  }
+  b->u.assign(ngrids, nullptr);
+  int st = FACES_OK;
+  for (int i = 0; i < ngrids && !st; i++) {
+    faces_plan *p = nullptr;
+    st = faces_plan_create(&p, 0, 1, 1, 1, mx[i], my[i], mz[i], n[i], boundary, 0);
+    if (!st) b->plans.push_back(p);
+  }
+  hipError_t e = hipSuccess;
+  if (!st) e = hipMalloc(&b->d_grids, sizeof(BatchGrid) * ngrids);
+  if (!st && e == hipSuccess) e = hipMalloc(&b->d_counter, 2 * sizeof(unsigned));
+  if (!st && e == hipSuccess) e = hipMemset(b->d_counter, 0, 2 * sizeof(unsigned));
+  if (!st && e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+  if (!st && e == hipSuccess) e = hipDeviceSynchronize();
+  if (!st && e != hipSuccess) st = hip_fail(e, "batch device state");
+  if (st) {
+    faces_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return FACES_OK;
+}
+
+int faces_batch_destroy(faces_batch *b)
+{
+  if (!b) return FACES_OK;
+  int st = FACES_OK;
+  for (faces_plan *p : b->plans) {
+    const int s = faces_plan_destroy(p);
+    if (s && !st) st = s;
+  }
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  if (b->d_grids) (void)hipFree(b->d_grids);
+  if (b->d_counter) (void)hipFree(b->d_counter);
+  delete b;
+  return st;
+}
+
+int faces_batch_set_fields(faces_batch *b, double *const *u)
+{
+  if (!b || !u) return fail(FACES_ERR_ARG, "null argument");
+  for (int i = 0; i < b->ngrids; i++)
+    if (!u[i]) return fail(FACES_ERR_ARG, "null field pointer for grid %d", i);
+  for (int i = 0; i < b->ngrids; i++) b->u[i] = u[i];
+  b->dirty = true;
+  return FACES_OK;
+}
+
+int faces_batch_set_tuning(faces_batch *b, int block, int seg_len, int blocks_per_cu)
+{
+  if (!b) return fail(FACES_ERR_ARG, "null batch");
+  if (block != 0 && block != kStreamThreads)
+    return fail(FACES_ERR_ARG, "batched kernel blocks have %d threads", kStreamThreads);
+  if (seg_len < 0 || seg_len > 4) return fail(FACES_ERR_ARG, "segment length %d outside [0,4]", seg_len);
+  if (blocks_per_cu < 0 || blocks_per_cu > 32)
+    return fail(FACES_ERR_ARG, "blocks per CU %d outside [0,32]", blocks_per_cu);
+  if (seg_len) b->seg_cap = seg_len;
+  if (blocks_per_cu) b->bpc = blocks_per_cu;
+  b->dirty = true;
+  return FACES_OK;
+}
+
+int faces_batch_share(faces_batch *b, int compute, void *stream)
+{
+  if (!b) return fail(FACES_ERR_ARG, "null batch");
+  if (!fields_bound(b)) return fail(FACES_ERR_STATE, "fields not bound (faces_batch_set_fields)");
+  if (b->boundary != FACES_BOUNDARY_PERIODIC || !compute)
+    return faces_batch_share_per_grid(b, compute, stream);  // open / surface-only: per grid
+  int st = ensure_graph(b);
+  if (st) return st;
+  if (b->exec) FACES_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  for (int i : b->solo) {
+    st = faces_share(b->plans[i], b->u[i], compute, stream);
+    if (st) return st;
+  }
+  return FACES_OK;
+}
+
+int faces_batch_share_per_grid(faces_batch *b, int compute, void *stream)
+{
+  if (!b) return fail(FACES_ERR_ARG, "null batch");
+  if (!fields_bound(b)) return fail(FACES_ERR_STATE, "fields not bound (faces_batch_set_fields)");
+  for (int i = 0; i < b->ngrids; i++) {
+    const int st = faces_share(b->plans[i], b->u[i], compute, stream);
+    if (st) return st;
+  }
+  return FACES_OK;
+}
+
+int faces_batch_info(const faces_batch *b, double *bytes, long *items)
+{
+  if (!b || !bytes || !items) return fail(FACES_ERR_ARG, "null argument");
+  double s = 0;
+  long it = 0;
+  for (int i = 0; i < b->ngrids; i++) {
+    s += 8.0 * faces_field_doubles(b->n[i], b->mx[i], b->my[i], b->mz[i], 0);
+    const int S = batchable(b->n[i]) ? batch_seg(b->n[i], b->mx[i], b->seg_cap) : 0;
+    it += S ? (long)b->my[i] * b->mz[i] * (b->mx[i] / S) : 0;
+  }
+  *bytes = s;
+  *items = it;
+  return FACES_OK;
+}
+
+int faces_batch_tune(faces_batch *b, int reps, void *stream, double *out)
+{
+  if (!b || !out) return fail(FACES_ERR_ARG, "null argument");
+  if (reps < 1) return fail(FACES_ERR_ARG, "reps must be positive");
+  if (!fields_bound(b)) return fail(FACES_ERR_STATE, "fields not bound (faces_batch_set_fields)");
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  FACES_HIP(hipEventCreate(&e0));
+  FACES_HIP(hipEventCreate(&e1));
+  double best = 1e30;
+  int best_seg = b->seg_cap, best_bpc = b->bpc;
+  int rc = FACES_OK;
+  for (int seg : {1, 2, 3, 4}) {
+    for (int bpc : {1, 2, 3, 4, 8}) {
+      b->seg_cap = seg;
+      b->bpc = bpc;
+      b->dirty = true;
+      rc = upload(b);
+      if (rc) break;
+      const BatchLaunch D = launch_desc(b);
+      const int nb = nblocks_of(b);
+      rc = launch_batch(D, nb, b->lds_bytes, st);  // warm-up
+      for (int r = 0; r < reps && !rc; r++) {
+        if (r == 0 && hipEventRecord(e0, st) != hipSuccess) rc = fail(FACES_ERR_HIP, "event");
+        if (!rc) rc = launch_batch(D, nb, b->lds_bytes, st);
+      }
+      if (rc) break;
+      float ms = 0;
+      if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(FACES_ERR_HIP, "timing events");
+        break;
+      }
+      const double us = 1e3 * ms / reps;
+      if (us < best) {
+        best = us;
+        best_seg = seg;
+        best_bpc = bpc;
+      }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->seg_cap = best_seg;
+  b->bpc = best_bpc;
+  b->dirty = true;
+  if (rc) return rc;
+  out[0] = kStreamThreads;
+  out[1] = best_seg;
+  out[2] = best_bpc;
+  out[3] = best;
+  return FACES_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+// faces_item.hpp -- one self-contained work item of the periodic DSS step:
+// S consecutive elements of one element row, loaded, summed in LDS and stored
+// back by one block.  Used by the batched kernel, whose blocks take items of
+// many grids (each with its own n and S) from a shared work list.
+//
+// The item's footprint is the one of the streaming kernel (faces_stream.hpp):
+// OWN planes iz < n-1 of its elements, the y partner rows, the z partner
+// plane, the yz partner row and the x seam column, laid out in LDS as
+// StreamShape<N,S> describes.  Every location of a grid is a copy of exactly
+// one node and each node is owned by exactly one item, so items of one grid
+// (and of different grids) never write the same location.
+#pragma once
+
+#include "faces_stream.hpp"
+
+namespace faces {
+
+// Per-grid constants of the batched step (device array, one per grid).
+struct BatchGrid {
+  double *u;
+  long ys, ps;           // jy and jz strides (doubles)
+  unsigned field_bytes;  // < 2^32 - 8
+  int n, mx, my, mz;
+  int S, nseg;           // elements per item, items per element row
+  int items;             // my*mz*nseg
+  int first;             // index of the grid's first item in the work list
+};
+
+template <int N, int S>
+__device__ __forceinline__ void stream_item(const BatchGrid &G, int it, double *lds)
+{
+  using SH = StreamShape<N, S>;
+  constexpr int n = N, K = N - 1, n2 = N * N, J = SH::J, JO = SH::JO, NN = SH::NN;
+  constexpr unsigned n3 = (unsigned)(N * N * N);
+  constexpr unsigned kOOB = 0xFFFFFFF8u;
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  const int tid = threadIdx.x;
+
+  const int sg = it % G.nseg;
+  const int row = it / G.nseg;
+  const int jy = row % G.my, jz = row / G.my;
+  const int jym = (jy == 0) ? G.my - 1 : jy - 1;
+  const int jzm = (jz == 0) ? G.mz - 1 : jz - 1;
+  const int jx0 = sg * S;
+  const int jxs = (jx0 == 0) ? G.mx - 1 : jx0 - 1;
+  const unsigned r[4] = {(unsigned)((jz * G.ps + jy * G.ys) * 8), (unsigned)((jz * G.ps + jym * G.ys) * 8),
+                         (unsigned)((jzm * G.ps + jy * G.ys) * 8), (unsigned)((jzm * G.ps + jym * G.ys) * 8)};
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(G.u, 0, (int)G.field_bytes, 0x00020000);
+
+  // footprint slot -> byte offset (kOOB for positions the item must not store)
+  unsigned ld[J], st[J];
+#pragma unroll
+  for (int j = 0; j < JO; j++) {
+    const int q = j * kStreamThreads + tid;
+    const unsigned b = r[0] + jx0 * n3 * 8;
+    ld[j] = b;
+    st[j] = kOOB;
+    if (q < SH::cOWN) {
+      const int e = q / (K * n2), rr = q - e * (K * n2);
+      const int ix = rr % n, iy = (rr / n) % n;
+      ld[j] = b + (e * n3 + rr) * 8;
+      if (iy < K && !(e == S - 1 && ix == K)) st[j] = ld[j];
+    }
+  }
+#pragma unroll
+  for (int jp = 0; jp < SH::JP; jp++) {
+    const int j = JO + jp;
+    int q = jp * kStreamThreads + tid;
+    unsigned off = r[0] + jx0 * n3 * 8;
+    bool keep = false;
+    if (q < SH::cY) {
+      const int e = q / (K * n), rr = q - e * (K * n);
+      const int iz = rr / n, ix = rr - (rr / n) * n;
+      off = r[1] + ((jx0 + e) * n3 + iz * n2 + K * n + ix) * 8;
+      keep = !(e == S - 1 && ix == K);
+    } else if ((q -= SH::cY) < SH::cZ) {
+      const int e = q / (K * n), rr = q - e * (K * n);
+      off = r[2] + ((jx0 + e) * n3 + K * n2 + rr) * 8;
+      keep = !(e == S - 1 && rr % n == K);
+    } else if ((q -= SH::cZ) < SH::cW) {
+      const int e = q / n, ix = q - (q / n) * n;
+      off = r[3] + ((jx0 + e) * n3 + K * n2 + K * n + ix) * 8;
+      keep = !(e == S - 1 && ix == K);
+    } else if ((q -= SH::cW) < SH::cS) {
+      keep = true;
+      const unsigned sb = jxs * n3;
+      if (q < K * K) off = r[0] + (sb + (q / K) * n2 + (q % K) * n + K) * 8;
+      else if (q < K * K + K) off = r[1] + (sb + (q - K * K) * n2 + K * n + K) * 8;
+      else if (q < K * K + 2 * K) off = r[2] + (sb + K * n2 + (q - K * K - K) * n + K) * 8;
+      else off = r[3] + (sb + K * n2 + K * n + K) * 8;
+    }
+    ld[j] = off;
+    st[j] = keep ? off : kOOB;
+  }
+
+  double v[J];
+#pragma unroll
+  for (int j = 0; j < J; j++)
+    v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, ld[j], 0, 0));
+#pragma unroll
+  for (int j = 0; j < J; j++) lds[j * kStreamThreads + tid] = v[j];
+  __syncthreads();
+
+  // shared nodes of the item's elements: x pairs, then y, then z
+#pragma unroll
+  for (int i = 0; i < NN; i++) {
+    const int k = i * kStreamThreads + tid;
+    if (k >= S * SH::nshared) continue;
+    const int e = k / SH::nshared;
+    int rr = k - e * SH::nshared;
+    int ix, iy, iz;
+    if (rr < K * K) {
+      iz = 0; iy = rr / K; ix = rr - (rr / K) * K;
+    } else if (rr < K * K + K * (K - 1)) {
+      rr -= K * K;
+      iz = 1 + rr / K; iy = 0; ix = rr - (rr / K) * K;
+    } else {
+      rr -= K * K + K * (K - 1);
+      constexpr int K1 = (K > 1) ? K - 1 : 1;
+      iz = 1 + rr / K1; iy = 1 + rr - (rr / K1) * K1; ix = 0;
+    }
+    const bool cx = ix == 0, cy = iy == 0, cz = iz == 0;
+    const int p0 = e * (K * n2) + iz * n2 + iy * n + ix;
+    const int p2 = SH::rY + e * (K * n) + iz * n + ix;
+    const int p4 = SH::rZ + e * (K * n) + iy * n + ix;
+    const int p6 = SH::rW + e * n + ix;
+    const int p1 = e > 0 ? (e - 1) * (K * n2) + iz * n2 + iy * n + K : SH::rS + iz * K + iy;
+    const int p3 = e > 0 ? SH::rY + (e - 1) * (K * n) + iz * n + K : SH::rS + K * K + iz;
+    const int p5 = e > 0 ? SH::rZ + (e - 1) * (K * n) + iy * n + K : SH::rS + K * K + K + iy;
+    const int p7 = e > 0 ? SH::rW + (e - 1) * n + K : SH::rS + K * K + 2 * K;
+    const double a0 = cx ? lds[p0] + lds[p1] : lds[p0];
+    double q0 = a0;
+    if (cy) q0 = a0 + (cx ? lds[p2] + lds[p3] : lds[p2]);
+    double res = q0;
+    if (cz) {
+      const double b0 = cx ? lds[p4] + lds[p5] : lds[p4];
+      double q1 = b0;
+      if (cy) q1 = b0 + (cx ? lds[p6] + lds[p7] : lds[p6]);
+      res = q0 + q1;
+    }
+    lds[p0] = res;
+    if (cx) lds[p1] = res;
+    if (cy) {
+      lds[p2] = res;
+      if (cx) lds[p3] = res;
+    }
+    if (cz) {
+      lds[p4] = res;
+      if (cx) lds[p5] = res;
+      if (cy) {
+        lds[p6] = res;
+        if (cx) lds[p7] = res;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < J; j++)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, lds[j * kStreamThreads + tid]),
+                                          rsrc, st[j], 0, 0);
+  __syncthreads();  // LDS is reused by the block's next item
+}
+
+}  // namespace faces

@@ -678,7 +678,7 @@ struct XchgDesc {
   int nslots;
 };
 
-__global__ __launch_bounds__(256) void k_exchange(const XchgDesc X, double *__restrict__ arena)
+static __global__ __launch_bounds__(256) void k_exchange(const XchgDesc X, double *__restrict__ arena)
 {
   const long total = X.idx_off[X.nslots];
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
@@ -707,7 +707,7 @@ __host__ __device__ inline uint64_t hash64(uint64_t seed, uint64_t idx)
 // kind 0: integer-valued in [0,1023]; 1: uniform [-1,1); 2: the reference's
 // positional init jz*1e4+jy*100+jx+iz*0.01+iy*1e-4+ix*1e-6 (gpu/main.cc:10-19),
 // evaluated left to right without contraction.
-__global__ __launch_bounds__(256) void k_fill(Geom G, int kind, uint64_t seed)
+static __global__ __launch_bounds__(256) void k_fill(Geom G, int kind, uint64_t seed)
 {
   const int n = G.n;
   const long n3 = (long)n * n * n;

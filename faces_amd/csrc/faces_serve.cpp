@@ -1,0 +1,411 @@
+// faces_serve.cpp -- the served step: a request loop with continuous batching.
+//
+// Requests (a dense device field and its shape) arrive at any time.  A worker
+// thread keeps up to `nslots` launches in flight, one per HIP stream: whenever
+// a slot is free it admits queued requests in arrival order -- up to
+// max_batch of them, while the bytes of admitted, unfinished requests stay
+// within the footprint cap -- and sends them as one batched launch (one work
+// list over all their element segments, faces_batch.hip) on that slot's
+// stream.  Requests that arrive while other launches run are admitted as soon
+// as a slot and footprint free up, so launches overlap on the device.  Every
+// request's result depends only on its own field (items never span grids), so
+// it is the same alone or in any mix.  Latency is measured from arrival to the
+// moment the worker observes the launch's completion event.
+//
+// No counterpart in the reference (north-star extension).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "faces_batch.hpp"
+#include "faces_common.hpp"
+
+using namespace faces;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double now_us(Clock::time_point t0) { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
+
+struct Request {
+  uint64_t ticket;
+  int n, mx, my, mz, boundary, compute;
+  double *u;
+  double bytes;
+  double t_arrival = 0, t_done = 0;
+  bool done = false;
+  int status = FACES_OK;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t event = nullptr;
+  BatchGrid *d_grids = nullptr;
+  BatchGrid *h_grids = nullptr;  // pinned staging
+  unsigned *d_counter = nullptr;
+  std::vector<Request *> reqs;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct faces_server {
+  double cap;
+  int max_batch;
+  int device = 0;
+  std::vector<Slot> slots;
+  Clock::time_point t0;
+
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request *> queue;
+  std::unordered_map<uint64_t, Request *> reqs;
+  uint64_t next_ticket = 1;
+  double admitted_bytes = 0, peak_bytes = 0;
+  long n_admitted = 0, n_completed = 0, n_launches = 0;
+  bool stop = false;
+  int worker_error = FACES_OK;
+  std::thread worker;
+
+  // plans for requests the batched kernel does not take (open boundary or
+  // compute = 0), keyed by shape; created by the worker on first use
+  std::map<std::tuple<int, int, int, int, int>, faces_plan *> plans;
+
+  void run();
+  int launch(Slot &s, std::vector<Request *> &batch);
+};
+
+int faces_server::launch(Slot &s, std::vector<Request *> &batch)
+{
+  std::vector<BatchGrid> g;
+  std::vector<Request *> other;
+  for (Request *r : batch) {
+    if (r->boundary == FACES_BOUNDARY_PERIODIC && r->compute && batch_seg(r->n, r->mx, 4)) {
+      BatchGrid x;
+      x.u = r->u;
+      x.n = r->n;
+      x.mx = r->mx;
+      x.my = r->my;
+      x.mz = r->mz;
+      x.ys = (long)x.n * x.n * x.n * x.mx;
+      x.ps = x.ys * x.my;
+      x.field_bytes = (unsigned)(8 * x.ps * x.mz);
+      g.push_back(x);
+    } else {
+      other.push_back(r);
+    }
+  }
+  if (!g.empty()) {
+    int lds = 0, nunits = 0;
+    int st = plan_grids(g, 4, &lds, &nunits);
+    if (st) return st;
+    std::copy(g.begin(), g.end(), s.h_grids);
+    FACES_HIP(hipMemcpyAsync(s.d_grids, s.h_grids, sizeof(BatchGrid) * g.size(), hipMemcpyHostToDevice,
+                             s.stream));
+    BatchLaunch D;
+    D.grids = s.d_grids;
+    D.ngrids = (int)g.size();
+    D.nunits = nunits;
+    D.counter = s.d_counter;
+    long nb = (long)device_cus() * 2;
+    if (nb > nunits) nb = nunits;
+    st = launch_batch(D, (int)std::max(1L, nb), lds, s.stream);
+    if (st) return st;
+  }
+  for (Request *r : other) {
+    auto key = std::make_tuple(r->n, r->mx, r->my, r->mz, r->boundary);
+    faces_plan *&p = plans[key];
+    if (!p) {
+      int st = faces_plan_create(&p, 0, 1, 1, 1, r->mx, r->my, r->mz, r->n, r->boundary, 0);
+      if (st) {
+        p = nullptr;
+        return st;
+      }
+    }
+    int st = faces_share(p, r->u, r->compute, s.stream);
+    if (st) return st;
+  }
+  FACES_HIP(hipEventRecord(s.event, s.stream));
+  s.reqs = batch;
+  s.busy = true;
+  n_launches++;
+  return FACES_OK;
+}
+
+void faces_server::run()
+{
+  if (hipSetDevice(device) != hipSuccess) {
+    std::lock_guard<std::mutex> lk(mu);
+    worker_error = fail(FACES_ERR_HIP, "server worker: hipSetDevice");
+    stop = true;
+    cv_done.notify_all();
+    return;
+  }
+  std::unique_lock<std::mutex> lk(mu);
+  while (true) {
+    // completions
+    bool any_busy = false;
+    for (Slot &s : slots) {
+      if (!s.busy) continue;
+      const hipError_t q = hipEventQuery(s.event);
+      if (q == hipSuccess) {
+        const double t = now_us(t0);
+        for (Request *r : s.reqs) {
+          r->t_done = t;
+          r->done = true;
+          admitted_bytes -= r->bytes;
+          n_completed++;
+        }
+        s.reqs.clear();
+        s.busy = false;
+        cv_done.notify_all();
+      } else if (q == hipErrorNotReady) {
+        any_busy = true;
+      } else {
+        for (Request *r : s.reqs) {
+          r->status = hip_fail(q, "server launch");
+          r->done = true;
+        }
+        s.reqs.clear();
+        s.busy = false;
+        cv_done.notify_all();
+      }
+    }
+    // admissions into free slots, in arrival order
+    for (Slot &s : slots) {
+      if (s.busy || queue.empty()) continue;
+      std::vector<Request *> batch;
+      while (!queue.empty() && (int)batch.size() < max_batch &&
+             admitted_bytes + queue.front()->bytes <= cap) {
+        Request *r = queue.front();
+        queue.pop_front();
+        admitted_bytes += r->bytes;
+        peak_bytes = std::max(peak_bytes, admitted_bytes);
+        n_admitted++;
+        batch.push_back(r);
+      }
+      if (batch.empty()) break;  // the head waits for footprint to free up
+      const int st = launch(s, batch);
+      if (st) {
+        for (Request *r : batch) {
+          r->status = st;
+          r->done = true;
+          admitted_bytes -= r->bytes;
+        }
+        cv_done.notify_all();
+      } else {
+        any_busy = true;
+      }
+    }
+    if (stop && queue.empty() && !any_busy) break;
+    if (any_busy) {
+      // launches in flight: poll their events without holding the lock long
+      lk.unlock();
+      std::this_thread::yield();
+      lk.lock();
+    } else {
+      cv_work.wait_for(lk, std::chrono::milliseconds(5));
+    }
+  }
+}
+
+extern "C" {
+
+int faces_server_create(faces_server **out, double footprint_cap, int max_batch, int nstreams)
+{
+  if (!out) return fail(FACES_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (!(footprint_cap > 0)) return fail(FACES_ERR_ARG, "footprint cap must be positive");
+  if (max_batch < 1 || max_batch > 65536) return fail(FACES_ERR_ARG, "max_batch %d outside [1,65536]", max_batch);
+  if (nstreams < 1 || nstreams > 16) return fail(FACES_ERR_ARG, "nstreams %d outside [1,16]", nstreams);
+  faces_server *s = new (std::nothrow) faces_server();
+  if (!s) return fail(FACES_ERR_NOMEM, "server allocation");
+  s->cap = footprint_cap;
+  s->max_batch = max_batch;
+  s->t0 = Clock::now();
+  hipError_t e = hipGetDevice(&s->device);
+  s->slots.resize(nstreams);
+  for (Slot &sl : s->slots) {
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.event, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc(&sl.d_grids, sizeof(BatchGrid) * max_batch);
+    if (e == hipSuccess) e = hipHostMalloc(&sl.h_grids, sizeof(BatchGrid) * max_batch, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&sl.d_counter, 2 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(sl.d_counter, 0, 2 * sizeof(unsigned));
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    const int st = hip_fail(e, "server device state");
+    faces_server_destroy(s);
+    return st;
+  }
+  s->worker = std::thread([s] { s->run(); });
+  *out = s;
+  return FACES_OK;
+}
+
+int faces_server_destroy(faces_server *s)
+{
+  if (!s) return FACES_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (Slot &sl : s->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.event) (void)hipEventDestroy(sl.event);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.d_grids) (void)hipFree(sl.d_grids);
+    if (sl.h_grids) (void)hipHostFree(sl.h_grids);
+    if (sl.d_counter) (void)hipFree(sl.d_counter);
+  }
+  for (auto &kv : s->plans) faces_plan_destroy(kv.second);
+  for (auto &kv : s->reqs) delete kv.second;
+  delete s;
+  return FACES_OK;
+}
+
+static int submit_at(faces_server *s, int n, int mx, int my, int mz, int boundary, int compute,
+                     double *u, double t_arrival, uint64_t *ticket)
+{
+  if (!s || !u || !ticket) return fail(FACES_ERR_ARG, "null argument");
+  if (n < 2 || n > 64 || mx < 1 || my < 1 || mz < 1) return fail(FACES_ERR_ARG, "bad shape");
+  if (boundary != FACES_BOUNDARY_OPEN && boundary != FACES_BOUNDARY_PERIODIC)
+    return fail(FACES_ERR_ARG, "unknown boundary %d", boundary);
+  const double bytes = 8.0 * faces_field_doubles(n, mx, my, mz, 0);
+  if (bytes > s->cap) return fail(FACES_ERR_ADMISSION, "request of %.0f bytes exceeds the cap %.0f", bytes, s->cap);
+  if (bytes >= 4294967280.0 && boundary == FACES_BOUNDARY_PERIODIC && compute)
+    return fail(FACES_ERR_ARG, "served fields are limited to 4 GB");
+  Request *r = new (std::nothrow) Request();
+  if (!r) return fail(FACES_ERR_NOMEM, "request allocation");
+  r->n = n;
+  r->mx = mx;
+  r->my = my;
+  r->mz = mz;
+  r->boundary = boundary;
+  r->compute = compute ? 1 : 0;
+  r->u = u;
+  r->bytes = bytes;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) {
+      delete r;
+      return fail(FACES_ERR_STATE, "server stopped");
+    }
+    r->ticket = s->next_ticket++;
+    r->t_arrival = t_arrival >= 0 ? t_arrival : now_us(s->t0);
+    s->reqs[r->ticket] = r;
+    s->queue.push_back(r);
+    *ticket = r->ticket;
+  }
+  s->cv_work.notify_one();
+  return FACES_OK;
+}
+
+int faces_server_submit(faces_server *s, int n, int mx, int my, int mz, int boundary, int compute,
+                        double *u, uint64_t *ticket)
+{
+  return submit_at(s, n, mx, my, mz, boundary, compute, u, -1.0, ticket);
+}
+
+int faces_server_wait(faces_server *s, uint64_t ticket, double timeout_s, double *latency_us)
+{
+  if (!s) return fail(FACES_ERR_ARG, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return fail(FACES_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+  Request *r = it->second;
+  const auto deadline = Clock::now() + std::chrono::duration<double>(timeout_s > 0 ? timeout_s : 1e9);
+  while (!r->done) {
+    if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && !r->done)
+      return fail(FACES_ERR_TIMEOUT, "request %llu not done", (unsigned long long)ticket);
+  }
+  const int st = r->status;
+  if (latency_us) *latency_us = r->t_done - r->t_arrival;
+  s->reqs.erase(it);
+  delete r;
+  if (st) return fail(st, "request failed: %s", faces_strerror(st));
+  return FACES_OK;
+}
+
+int faces_server_run_trace(faces_server *s, int nreq, const double *arrival_us, const int *n,
+                           const int *mx, const int *my, const int *mz, int boundary, int compute,
+                           double *const *u, double *latency_us, double *stats)
+{
+  if (!s || nreq < 1 || !arrival_us || !n || !mx || !my || !mz || !u || !latency_us || !stats)
+    return fail(FACES_ERR_ARG, "null argument");
+  for (int i = 1; i < nreq; i++)
+    if (arrival_us[i] < arrival_us[i - 1]) return fail(FACES_ERR_ARG, "arrivals must be sorted");
+  std::vector<uint64_t> tickets(nreq, 0);
+  long launches0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    launches0 = s->n_launches;
+  }
+  const double start = now_us(s->t0);
+  int st = FACES_OK;
+  for (int i = 0; i < nreq && !st; i++) {
+    const double due = start + arrival_us[i];
+    while (now_us(s->t0) < due) {
+      if (due - now_us(s->t0) > 200.0) std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+    st = submit_at(s, n[i], mx[i], my[i], mz[i], boundary, compute, u[i], due, &tickets[i]);
+  }
+  double last = start;
+  for (int i = 0; i < nreq; i++) {
+    if (!tickets[i]) continue;
+    double lat = 0;
+    const int w = faces_server_wait(s, tickets[i], 600.0, &lat);
+    if (w && !st) st = w;
+    latency_us[i] = lat;
+    last = std::max(last, start + arrival_us[i] + lat);
+  }
+  if (st) return st;
+  std::vector<double> sorted(latency_us, latency_us + nreq);
+  std::sort(sorted.begin(), sorted.end());
+  auto pct = [&](double p) {
+    const double x = p * (nreq - 1);
+    const int i0 = (int)x;
+    const int i1 = std::min(nreq - 1, i0 + 1);
+    return sorted[i0] + (x - i0) * (sorted[i1] - sorted[i0]);
+  };
+  const double wall = (last - start) * 1e-6;
+  stats[0] = wall;
+  stats[1] = wall > 0 ? nreq / wall : 0;
+  stats[2] = pct(0.50);
+  stats[3] = pct(0.99);
+  stats[4] = sorted[nreq - 1];
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    stats[5] = (double)(s->n_launches - launches0);
+  }
+  return FACES_OK;
+}
+
+int faces_server_stats(const faces_server *s, double *stats)
+{
+  if (!s || !stats) return fail(FACES_ERR_ARG, "null argument");
+  faces_server *m = const_cast<faces_server *>(s);
+  std::lock_guard<std::mutex> lk(m->mu);
+  stats[0] = (double)m->n_admitted;
+  stats[1] = (double)m->n_completed;
+  stats[2] = (double)m->n_launches;
+  stats[3] = m->peak_bytes;
+  return FACES_OK;
+}
+
+}  // extern "C"
