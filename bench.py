@@ -10,6 +10,16 @@ region holds only the share launches.
 value = algorithmic bytes per step / step time, with algorithmic bytes
 B_step = 16 * mx*my*mz*n^3 (the field read once and written once).
 
+Two further legs ride on the same line:
+  "batched": 256 independent periodic grids (mx, my, mz uniform in [2,24], n
+  from {4,6,8}, integer-valued [0,1023]) stepped by one persistent launch
+  replayed from a HIP graph after the autotuner picked its shape, against the
+  same grids stepped one launch per grid: grids/s, GB/s, and their ratio.
+  "served": Poisson arrivals (mean 20k requests/s) of requests sized as the
+  batched grids plus 5% large ones (48^3 elements, n=7), random FP64 [-1,1),
+  fields resident in HBM, admission capped at 200 GB: p50/p99 latency and
+  requests/s.
+
 Prints ONE JSON line.  --gpus above 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -54,7 +64,117 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-kernel reference")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
+    ap.add_argument("--grids", type=int, default=256, help="grids in the batched leg")
+    ap.add_argument("--requests", type=int, default=2000, help="requests in the served leg")
     return ap.parse_args()
+
+
+def splitmix64(seed, idx):
+    """splitmix64 finaliser of seed ^ (idx * golden ratio): seeded shapes."""
+    m = (1 << 64) - 1
+    z = (seed ^ ((idx * 0x9E3779B97F4A7C15) & m)) & m
+    z = (z + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def mixed_shapes(count, seed):
+    """(n, mx, my, mz): mx, my, mz uniform in [2,24], n from {4,6,8}."""
+    out = []
+    for g in range(count):
+        h = [splitmix64(seed, (g << 3) + k) for k in range(4)]
+        out.append(((4, 6, 8)[h[3] % 3], 2 + h[0] % 23, 2 + h[1] % 23, 2 + h[2] % 23))
+    return out
+
+
+def event_ms(torch, fn, reps, stream):
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record(stream)
+    for _ in range(reps):
+        fn()
+    e.record(stream)
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def batched_leg(torch, fa, ngrids, steps, warmup):
+    from faces_amd.batch import Batch
+    shapes = mixed_shapes(ngrids, seed=4)
+    fields = [torch.empty(fa.field_doubles(*sh), dtype=torch.float64, device="cuda")
+              for sh in shapes]
+
+    def refill():
+        for i, (d, sh) in enumerate(zip(fields, shapes)):
+            fa.fill(d, *sh, kind=fa.FILL_INT, seed=42 + i)
+        torch.cuda.synchronize()
+
+    refill()
+    b = Batch(shapes)
+    b.set_fields(fields)
+    stream = torch.cuda.current_stream()
+    tuned = b.tune(reps=5, stream=stream)
+    refill()
+    for _ in range(warmup):
+        b.share(stream=stream)
+        b.share_per_grid(stream=stream)
+    ms_b = event_ms(torch, lambda: b.share(stream=stream), steps, stream)
+    ms_p = event_ms(torch, lambda: b.share_per_grid(stream=stream), max(5, steps // 4), stream)
+    nbytes = sum(16.0 * n ** 3 * mx * my * mz for n, mx, my, mz in shapes)
+    b.close()
+    del fields
+    torch.cuda.synchronize()
+    return {"workload": f"{ngrids} periodic grids, mx,my,mz uniform [2,24], n from {{4,6,8}}, "
+                        "integer-valued [0,1023]; one persistent work-list launch per step "
+                        "replayed from a HIP graph",
+            "grids": ngrids, "bytes_per_step": int(nbytes),
+            "ms_per_step": round(ms_b, 5), "grids_per_s": round(ngrids / (ms_b * 1e-3), 1),
+            "GBs": round(nbytes / (ms_b * 1e-3) / 1e9, 2),
+            "per_grid_ms_per_step": round(ms_p, 5),
+            "per_grid_grids_per_s": round(ngrids / (ms_p * 1e-3), 1),
+            "per_grid_GBs": round(nbytes / (ms_p * 1e-3) / 1e9, 2),
+            "speedup_vs_per_grid": round(ms_p / ms_b, 3),
+            "tuned": {"seg_len": tuned["seg_len"], "blocks_per_cu": tuned["blocks_per_cu"]}}
+
+
+def served_leg(torch, fa, nreq):
+    import random
+    from faces_amd.batch import Server
+    rng = random.Random(20)
+    small = mixed_shapes(nreq, seed=5)
+    shapes, arrivals, t = [], [], 0.0
+    for i in range(nreq):
+        t += rng.expovariate(1.0 / 50.0)  # mean 50 us between arrivals: 20k requests/s
+        arrivals.append(t)
+        shapes.append((7, 48, 48, 48) if rng.random() < 0.05 else small[i])
+    fields = []
+    for i, sh in enumerate(shapes):
+        d = torch.empty(fa.field_doubles(*sh), dtype=torch.float64, device="cuda")
+        fa.fill(d, *sh, kind=fa.FILL_RAND, seed=7000 + i)
+        fields.append(d)
+    torch.cuda.synchronize()
+    cap = 200e9
+    srv = Server(footprint_cap=cap, max_batch=256, nstreams=2)
+    # warm the server's launch path on a few requests first
+    warm = srv.run_trace([0.0, 10.0, 20.0], shapes[:3], fields[:3])
+    del warm
+    lat, st = srv.run_trace(arrivals, shapes, fields)
+    srv.close()
+    nbytes = sum(16.0 * n ** 3 * mx * my * mz for n, mx, my, mz in shapes)
+    nlarge = sum(1 for sh in shapes if sh == (7, 48, 48, 48))
+    del fields
+    torch.cuda.synchronize()
+    return {"workload": f"{nreq} requests, Poisson arrivals mean 20k req/s, sizes as the batched "
+                        f"grids plus 5% of 48^3 n=7 ({nlarge} drawn), random FP64 [-1,1), "
+                        "fields resident in HBM, admission cap 200 GB",
+            "requests": nreq, "offered_req_per_s": round(nreq / (arrivals[-1] * 1e-6), 1),
+            "req_per_s": round(st["req_per_s"], 1), "p50_us": round(st["p50_us"], 1),
+            "p99_us": round(st["p99_us"], 1), "max_us": round(st["max_us"], 1),
+            "launches": int(st["launches"]), "footprint_cap_bytes": cap,
+            "GBs": round(nbytes / st["wall_s"] / 1e9, 2)}
 
 
 def copy_reference_gbs(torch, nbytes, reps=20):
@@ -159,6 +279,9 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(n, mx, my, mz, kind, args.cpu_seconds)
 
+    batched = None if args.no_batch else batched_leg(torch, fa, args.grids, 50, 5)
+    served = None if args.no_serve else served_leg(torch, fa, args.requests)
+
     per_sorted = sorted(per)
     rec = {
         "metric": METRIC,
@@ -186,6 +309,8 @@ def main():
         "copy_reference_gbs": None if copy_gbs is None else round(copy_gbs, 1),
         "frac_of_copy": None if copy_gbs is None else round(gbs / copy_gbs, 4),
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(rec), flush=True)
 

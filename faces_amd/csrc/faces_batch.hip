@@ -3,6 +3,7 @@
 // element-segment items, replayed from a captured HIP graph.  No counterpart
 // in the reference, which steps one grid per process (gpu/main.cc:170-186).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -16,16 +17,20 @@ namespace faces {
 // ---------------------------------------------------------------------------
 // the batched kernel
 // ---------------------------------------------------------------------------
-// Blocks take items from a shared counter; the item's grid is found by binary
-// search over the grids' first-item offsets.  The block that finishes last
-// resets the counter, so consecutive launches (and graph replays) need no
-// memset.  Counters are updated with vector atomics.
+// Static schedule, no atomics: the work list is cut into one range per XCD
+// (equal estimated cost), and the blocks that the hardware places on XCD x
+// (blockIdx % 8 == x, round-robin dispatch) stride through range x.  Items
+// that touch the same sectors (neighbouring segments, partner rows) are then
+// processed at about the same time under one L2, where partial-sector writes
+// of different blocks merge before they reach HBM.  The item's grid is found
+// by walking forward from the previous item's grid.
 
-#define FACES_BATCH_CASES(X)                                                              \
-  X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(3, 1) X(3, 2) X(3, 3) X(3, 4) X(4, 1) X(4, 2) X(4, 3) \
-  X(4, 4) X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(6, 1) X(6, 2) X(6, 3) X(6, 4) X(7, 1) X(7, 2) \
-  X(7, 3) X(7, 4) X(8, 1) X(8, 2) X(8, 3) X(8, 4) X(9, 1) X(9, 2) X(9, 3) X(10, 1)        \
-  X(10, 2) X(11, 1) X(11, 2) X(12, 1)
+// (n, S) routines: per n, S from 1 up to about 2K points per item.
+#define FACES_BATCH_CASES(X)                                                                  \
+  X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(2, 16) X(2, 32) X(3, 1) X(3, 2) X(3, 4) X(3, 8) X(3, 16)  \
+  X(4, 1) X(4, 2) X(4, 4) X(4, 8) X(4, 16) X(5, 1) X(5, 2) X(5, 4) X(5, 8) X(6, 1) X(6, 2)   \
+  X(6, 4) X(6, 8) X(7, 1) X(7, 2) X(7, 4) X(7, 6) X(8, 1) X(8, 2) X(8, 4) X(9, 1) X(9, 2)   \
+  X(9, 3) X(10, 1) X(10, 2) X(11, 1) X(11, 2) X(12, 1)
 
 // One out-of-line routine per (n, S): inlining all of them into the kernel's
 // switch makes the register allocator size every path for the union of their
@@ -39,42 +44,30 @@ __device__ __noinline__ void batch_item(const BatchGrid G, int it, double *lds)
 __global__ __launch_bounds__(kStreamThreads) void k_batch(BatchLaunch D)
 {
   extern __shared__ double lds[];
-  __shared__ int s_unit;
-  while (true) {
-    if (threadIdx.x == 0) s_unit = (int)atomicAdd(&D.counter[0], 1u);
-    __syncthreads();
-    const int u = s_unit;
-    __syncthreads();  // all threads have read s_unit before it is overwritten
-    if (u >= D.nunits) break;
-    int lo = 0, hi = D.ngrids - 1;  // last grid with first <= u
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (D.grids[mid].first <= u) lo = mid; else hi = mid - 1;
-    }
-    const BatchGrid G = D.grids[lo];
+  const int x = blockIdx.x % kXcds;
+  const int stride = gridDim.x / kXcds;
+  const int end = D.xr[x + 1];
+  int g = 0;
+  for (int u = D.xr[x] + (int)(blockIdx.x / kXcds); u < end; u += stride) {
+    while (g + 1 < D.ngrids && D.grids[g + 1].first <= u) g++;
+    const BatchGrid G = D.grids[g];
     const int it = u - G.first;
-    switch (G.n * 8 + G.S) {
+    switch (G.n * 64 + G.S) {
 #define FACES_BATCH_CASE(NN, SS) \
-  case NN * 8 + SS: batch_item<NN, SS>(G, it, lds); break;
+  case NN * 64 + SS: batch_item<NN, SS>(G, it, lds); break;
       FACES_BATCH_CASES(FACES_BATCH_CASE)
 #undef FACES_BATCH_CASE
       default: break;  // excluded on the host
-    }
-  }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    if (atomicAdd(&D.counter[1], 1u) == gridDim.x - 1) {
-      atomicExch(&D.counter[0], 0u);
-      atomicExch(&D.counter[1], 0u);
     }
   }
 }
 
 bool batch_case_ok(int n, int S)
 {
-  switch (n * 8 + S) {
+  if (S < 1 || S > 63) return false;
+  switch (n * 64 + S) {
 #define FACES_BATCH_CASE(NN, SS) \
-  case NN * 8 + SS: return true;
+  case NN * 64 + SS: return true;
     FACES_BATCH_CASES(FACES_BATCH_CASE)
 #undef FACES_BATCH_CASE
     default: return false;
@@ -90,20 +83,41 @@ int batch_lds_bytes(int n, int S)
 
 int batch_seg(int n, int mx, int cap)
 {
-  for (int s = std::min(cap, 4); s >= 1; s--)
-    if (mx % s == 0 && batch_case_ok(n, s) && batch_lds_bytes(n, s) <= 64 * 1024) return s;
-  return 0;
+  // the smallest routine that holds the whole row, else the largest one
+  int best = 0;
+  for (int s = 1; s <= std::min(cap, 63); s++) {
+    if (!batch_case_ok(n, s) || batch_lds_bytes(n, s) > 64 * 1024) continue;
+    best = s;
+    if (s >= mx) break;
+  }
+  return best;
+}
+
+int batch_blocks(int nunits, int blocks_per_cu)
+{
+  long nb = (long)device_cus() * blocks_per_cu;
+  nb = std::min(nb, (long)nunits + kXcds - 1);
+  nb = (nb + kXcds - 1) / kXcds * kXcds;
+  return (int)std::max((long)kXcds, nb);
 }
 
 int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t st)
 {
   if (D.nunits == 0) return FACES_OK;
+  if (nblocks % kXcds) return fail(FACES_ERR_ARG, "batched launch of %d blocks", nblocks);
   k_batch<<<dim3(nblocks), dim3(kStreamThreads), lds_bytes, st>>>(D);
   FACES_HIP(hipGetLastError());
   return FACES_OK;
 }
 
-int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits)
+// Estimated cost of one item of a grid, in doubles moved: the segment's
+// points plus a fixed per-item latency share.
+static double item_cost(const BatchGrid &x)
+{
+  return (double)std::min(x.S, x.mx) * x.n * x.n * x.n + 512.0;
+}
+
+int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, int *xr)
 {
   // Items of expensive grids first: the work list then ends on cheap items.
   std::vector<int> order(g.size());
@@ -111,7 +125,7 @@ int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits)
   for (auto &x : g) {
     x.S = batch_seg(x.n, x.mx, cap);
     if (x.S == 0) return fail(FACES_ERR_ARG, "grid n=%d mx=%d has no batched item shape", x.n, x.mx);
-    x.nseg = x.mx / x.S;
+    x.nseg = (x.mx + x.S - 1) / x.S;
     x.items = x.my * x.mz * x.nseg;
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -129,6 +143,22 @@ int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits)
   g.swap(sorted);
   *lds_bytes = lds;
   *nunits = first;
+  // XCD ranges of equal cost (cut at item granularity)
+  double total = 0;
+  for (const auto &x : g) total += x.items * item_cost(x);
+  xr[0] = 0;
+  int k = 1;
+  double acc = 0;
+  for (const auto &x : g) {
+    const double c = item_cost(x);
+    while (k < kXcds && acc + x.items * c >= total * k / kXcds) {
+      const long cut = x.first + (long)std::ceil((total * k / kXcds - acc) / c);
+      xr[k] = (int)std::min<long>(std::max<long>(cut, xr[k - 1]), x.first + x.items);
+      k++;
+    }
+    acc += x.items * c;
+  }
+  while (k <= kXcds) xr[k++] = first;
   return FACES_OK;
 }
 
@@ -145,10 +175,10 @@ struct faces_batch {
   std::vector<int> solo;            // grids without a batched item shape (n > 12): own plan
   // device state
   BatchGrid *d_grids = nullptr;
-  unsigned *d_counter = nullptr;
+  int xr[kXcds + 1] = {};
   int lds_bytes = 0, nunits = 0, nbatched = 0;
   // tuning
-  int seg_cap = 4, bpc = 2;
+  int seg_cap = 64, bpc = 4;
   // captured graph of the batched launch
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -183,7 +213,7 @@ int upload(faces_batch *b)
   b->lds_bytes = 0;
   b->nunits = 0;
   if (!g.empty()) {
-    int st = plan_grids(g, b->seg_cap, &b->lds_bytes, &b->nunits);
+    int st = plan_grids(g, b->seg_cap, &b->lds_bytes, &b->nunits, b->xr);
     if (st) return st;
     // launches of the old descriptors may still run on any stream
     FACES_HIP(hipDeviceSynchronize());
@@ -208,16 +238,11 @@ BatchLaunch launch_desc(const faces_batch *b)
   D.grids = b->d_grids;
   D.ngrids = b->nbatched;
   D.nunits = b->nunits;
-  D.counter = b->d_counter;
+  std::copy(b->xr, b->xr + kXcds + 1, D.xr);
   return D;
 }
 
-int nblocks_of(const faces_batch *b)
-{
-  long nb = (long)device_cus() * b->bpc;
-  if (nb > b->nunits) nb = b->nunits;
-  return (int)std::max(1L, nb);
-}
+int nblocks_of(const faces_batch *b) { return batch_blocks(b->nunits, b->bpc); }
 
 int ensure_graph(faces_batch *b)
 {
@@ -295,8 +320,6 @@ int faces_batch_create(faces_batch **out, int ngrids, const int *n, const int *m
   }
   hipError_t e = hipSuccess;
   if (!st) e = hipMalloc(&b->d_grids, sizeof(BatchGrid) * ngrids);
-  if (!st && e == hipSuccess) e = hipMalloc(&b->d_counter, 2 * sizeof(unsigned));
-  if (!st && e == hipSuccess) e = hipMemset(b->d_counter, 0, 2 * sizeof(unsigned));
   if (!st && e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
   if (!st && e == hipSuccess) e = hipDeviceSynchronize();
   if (!st && e != hipSuccess) st = hip_fail(e, "batch device state");
@@ -320,7 +343,6 @@ int faces_batch_destroy(faces_batch *b)
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   if (b->d_grids) (void)hipFree(b->d_grids);
-  if (b->d_counter) (void)hipFree(b->d_counter);
   delete b;
   return st;
 }
@@ -340,7 +362,7 @@ int faces_batch_set_tuning(faces_batch *b, int block, int seg_len, int blocks_pe
   if (!b) return fail(FACES_ERR_ARG, "null batch");
   if (block != 0 && block != kStreamThreads)
     return fail(FACES_ERR_ARG, "batched kernel blocks have %d threads", kStreamThreads);
-  if (seg_len < 0 || seg_len > 4) return fail(FACES_ERR_ARG, "segment length %d outside [0,4]", seg_len);
+  if (seg_len < 0 || seg_len > 64) return fail(FACES_ERR_ARG, "segment length %d outside [0,64]", seg_len);
   if (blocks_per_cu < 0 || blocks_per_cu > 32)
     return fail(FACES_ERR_ARG, "blocks per CU %d outside [0,32]", blocks_per_cu);
   if (seg_len) b->seg_cap = seg_len;
@@ -384,7 +406,7 @@ int faces_batch_info(const faces_batch *b, double *bytes, long *items)
   for (int i = 0; i < b->ngrids; i++) {
     s += 8.0 * faces_field_doubles(b->n[i], b->mx[i], b->my[i], b->mz[i], 0);
     const int S = batchable(b->n[i]) ? batch_seg(b->n[i], b->mx[i], b->seg_cap) : 0;
-    it += S ? (long)b->my[i] * b->mz[i] * (b->mx[i] / S) : 0;
+    it += S ? (long)b->my[i] * b->mz[i] * ((b->mx[i] + S - 1) / S) : 0;
   }
   *bytes = s;
   *items = it;
@@ -403,8 +425,8 @@ int faces_batch_tune(faces_batch *b, int reps, void *stream, double *out)
   double best = 1e30;
   int best_seg = b->seg_cap, best_bpc = b->bpc;
   int rc = FACES_OK;
-  for (int seg : {1, 2, 3, 4}) {
-    for (int bpc : {1, 2, 3, 4, 8}) {
+  for (int seg : {2, 4, 8, 16, 64}) {
+    for (int bpc : {1, 2, 3, 4, 6, 8}) {
       b->seg_cap = seg;
       b->bpc = bpc;
       b->dirty = true;

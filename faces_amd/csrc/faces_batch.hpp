@@ -10,12 +10,14 @@
 
 namespace faces {
 
-// One batched launch: grids sorted by first item, a work counter pair.
+// One batched launch: grids sorted by first item; items [xr[x], xr[x+1]) go
+// to the blocks that the hardware dispatches to XCD x (blocks x, x+8, ...).
+constexpr int kXcds = 8;
 struct BatchLaunch {
   const BatchGrid *grids;
   int ngrids;
   int nunits;
-  unsigned *counter;  // [0] next item, [1] finished blocks (zero between launches)
+  int xr[kXcds + 1];
 };
 
 // Elements per item for a grid (largest S <= cap dividing mx with a kernel
@@ -23,11 +25,15 @@ struct BatchLaunch {
 int batch_seg(int n, int mx, int cap);
 int batch_lds_bytes(int n, int S);
 
-// Fills S/nseg/items, sorts the grids by item cost and assigns first-item
-// offsets; returns the LDS bytes and item count of the launch.
-int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits);
+// Fills S/nseg/items, sorts the grids by item cost, assigns first-item
+// offsets and splits the work list into kXcds ranges of equal estimated
+// cost; returns the LDS bytes and item count of the launch.
+int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, int *xr);
 
-// Enqueues the batched kernel (nblocks persistent blocks).
+// Persistent blocks for a launch: blocks_per_cu per CU, a multiple of kXcds.
+int batch_blocks(int nunits, int blocks_per_cu);
+
+// Enqueues the batched kernel (nblocks persistent blocks, a multiple of kXcds).
 int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t st);
 
 }  // namespace faces

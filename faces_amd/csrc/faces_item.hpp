@@ -9,6 +9,9 @@
 // StreamShape<N,S> describes.  Every location of a grid is a copy of exactly
 // one node and each node is owned by exactly one item, so items of one grid
 // (and of different grids) never write the same location.
+//
+// The last item of a row may hold fewer than S elements (mx need not be a
+// multiple of S): slots of absent elements load and store nothing.
 #pragma once
 
 #include "faces_stream.hpp"
@@ -21,7 +24,7 @@ struct BatchGrid {
   long ys, ps;           // jy and jz strides (doubles)
   unsigned field_bytes;  // < 2^32 - 8
   int n, mx, my, mz;
-  int S, nseg;           // elements per item, items per element row
+  int S, nseg;           // elements per item (the row's last item may hold fewer), items per row
   int items;             // my*mz*nseg
   int first;             // index of the grid's first item in the work list
 };
@@ -43,6 +46,7 @@ __device__ __forceinline__ void stream_item(const BatchGrid &G, int it, double *
   const int jzm = (jz == 0) ? G.mz - 1 : jz - 1;
   const int jx0 = sg * S;
   const int jxs = (jx0 == 0) ? G.mx - 1 : jx0 - 1;
+  const int ecnt = min(S, G.mx - jx0);  // elements present in this item
   const unsigned r[4] = {(unsigned)((jz * G.ps + jy * G.ys) * 8), (unsigned)((jz * G.ps + jym * G.ys) * 8),
                          (unsigned)((jzm * G.ps + jy * G.ys) * 8), (unsigned)((jzm * G.ps + jym * G.ys) * 8)};
   const __amdgpu_buffer_rsrc_t rsrc =
@@ -59,8 +63,8 @@ __device__ __forceinline__ void stream_item(const BatchGrid &G, int it, double *
     if (q < SH::cOWN) {
       const int e = q / (K * n2), rr = q - e * (K * n2);
       const int ix = rr % n, iy = (rr / n) % n;
-      ld[j] = b + (e * n3 + rr) * 8;
-      if (iy < K && !(e == S - 1 && ix == K)) st[j] = ld[j];
+      ld[j] = e < ecnt ? b + (e * n3 + rr) * 8 : kOOB;
+      if (e < ecnt && iy < K && !(e == ecnt - 1 && ix == K)) st[j] = ld[j];
     }
   }
 #pragma unroll
@@ -72,16 +76,16 @@ __device__ __forceinline__ void stream_item(const BatchGrid &G, int it, double *
     if (q < SH::cY) {
       const int e = q / (K * n), rr = q - e * (K * n);
       const int iz = rr / n, ix = rr - (rr / n) * n;
-      off = r[1] + ((jx0 + e) * n3 + iz * n2 + K * n + ix) * 8;
-      keep = !(e == S - 1 && ix == K);
+      off = e < ecnt ? r[1] + ((jx0 + e) * n3 + iz * n2 + K * n + ix) * 8 : kOOB;
+      keep = e < ecnt && !(e == ecnt - 1 && ix == K);
     } else if ((q -= SH::cY) < SH::cZ) {
       const int e = q / (K * n), rr = q - e * (K * n);
-      off = r[2] + ((jx0 + e) * n3 + K * n2 + rr) * 8;
-      keep = !(e == S - 1 && rr % n == K);
+      off = e < ecnt ? r[2] + ((jx0 + e) * n3 + K * n2 + rr) * 8 : kOOB;
+      keep = e < ecnt && !(e == ecnt - 1 && rr % n == K);
     } else if ((q -= SH::cZ) < SH::cW) {
       const int e = q / n, ix = q - (q / n) * n;
-      off = r[3] + ((jx0 + e) * n3 + K * n2 + K * n + ix) * 8;
-      keep = !(e == S - 1 && ix == K);
+      off = e < ecnt ? r[3] + ((jx0 + e) * n3 + K * n2 + K * n + ix) * 8 : kOOB;
+      keep = e < ecnt && !(e == ecnt - 1 && ix == K);
     } else if ((q -= SH::cW) < SH::cS) {
       keep = true;
       const unsigned sb = jxs * n3;
@@ -106,7 +110,7 @@ __device__ __forceinline__ void stream_item(const BatchGrid &G, int it, double *
 #pragma unroll
   for (int i = 0; i < NN; i++) {
     const int k = i * kStreamThreads + tid;
-    if (k >= S * SH::nshared) continue;
+    if (k >= ecnt * SH::nshared) continue;
     const int e = k / SH::nshared;
     int rr = k - e * SH::nshared;
     int ix, iy, iz;

@@ -12,7 +12,7 @@
 // it is the same alone or in any mix.  Latency is measured from arrival to the
 // moment the worker observes the launch's completion event.
 //
-// No counterpart in the reference (north-star extension).
+// No counterpart in the reference, which steps one grid per process.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,7 +54,6 @@ struct Slot {
   hipEvent_t event = nullptr;
   BatchGrid *d_grids = nullptr;
   BatchGrid *h_grids = nullptr;  // pinned staging
-  unsigned *d_counter = nullptr;
   std::vector<Request *> reqs;
   bool busy = false;
 };
@@ -92,7 +91,7 @@ int faces_server::launch(Slot &s, std::vector<Request *> &batch)
   std::vector<BatchGrid> g;
   std::vector<Request *> other;
   for (Request *r : batch) {
-    if (r->boundary == FACES_BOUNDARY_PERIODIC && r->compute && batch_seg(r->n, r->mx, 4)) {
+    if (r->boundary == FACES_BOUNDARY_PERIODIC && r->compute && batch_seg(r->n, r->mx, 64)) {
       BatchGrid x;
       x.u = r->u;
       x.n = r->n;
@@ -109,7 +108,8 @@ int faces_server::launch(Slot &s, std::vector<Request *> &batch)
   }
   if (!g.empty()) {
     int lds = 0, nunits = 0;
-    int st = plan_grids(g, 4, &lds, &nunits);
+    int xr[kXcds + 1];
+    int st = plan_grids(g, 64, &lds, &nunits, xr);
     if (st) return st;
     std::copy(g.begin(), g.end(), s.h_grids);
     FACES_HIP(hipMemcpyAsync(s.d_grids, s.h_grids, sizeof(BatchGrid) * g.size(), hipMemcpyHostToDevice,
@@ -118,10 +118,8 @@ int faces_server::launch(Slot &s, std::vector<Request *> &batch)
     D.grids = s.d_grids;
     D.ngrids = (int)g.size();
     D.nunits = nunits;
-    D.counter = s.d_counter;
-    long nb = (long)device_cus() * 2;
-    if (nb > nunits) nb = nunits;
-    st = launch_batch(D, (int)std::max(1L, nb), lds, s.stream);
+    std::copy(xr, xr + kXcds + 1, D.xr);
+    st = launch_batch(D, batch_blocks(nunits, 4), lds, s.stream);
     if (st) return st;
   }
   for (Request *r : other) {
@@ -242,8 +240,6 @@ int faces_server_create(faces_server **out, double footprint_cap, int max_batch,
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.event, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&sl.d_grids, sizeof(BatchGrid) * max_batch);
     if (e == hipSuccess) e = hipHostMalloc(&sl.h_grids, sizeof(BatchGrid) * max_batch, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc(&sl.d_counter, 2 * sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemset(sl.d_counter, 0, 2 * sizeof(unsigned));
   }
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
@@ -271,7 +267,6 @@ int faces_server_destroy(faces_server *s)
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.d_grids) (void)hipFree(sl.d_grids);
     if (sl.h_grids) (void)hipHostFree(sl.h_grids);
-    if (sl.d_counter) (void)hipFree(sl.d_counter);
   }
   for (auto &kv : s->plans) faces_plan_destroy(kv.second);
   for (auto &kv : s->reqs) delete kv.second;

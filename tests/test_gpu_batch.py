@@ -55,11 +55,29 @@ def assert_all_same(got, want, shapes):
         assert r["ndiff"] == 0, (i, s, r)
 
 
-# every (n, S) routine of the batched kernel at least once, with ragged rows
-EDGE_SHAPES = [(2, 1, 1, 1), (2, 4, 2, 3), (3, 3, 1, 2), (3, 8, 2, 1), (4, 6, 3, 2), (4, 5, 1, 1),
-               (5, 12, 2, 2), (5, 7, 3, 1), (6, 2, 2, 2), (6, 9, 1, 3), (7, 4, 3, 2), (7, 3, 2, 2),
-               (7, 1, 1, 1), (8, 8, 2, 2), (8, 6, 1, 1), (9, 3, 2, 1), (9, 2, 1, 2), (10, 2, 2, 1),
-               (10, 1, 1, 2), (11, 2, 1, 1), (11, 3, 1, 1), (12, 1, 2, 1), (12, 2, 1, 1)]
+# (n, S) routines of the batched kernel; a grid takes the smallest S that
+# holds its whole row, else the largest (and then a shorter last item)
+ROUTINES = {2: [1, 2, 4, 8, 16, 32], 3: [1, 2, 4, 8, 16], 4: [1, 2, 4, 8, 16], 5: [1, 2, 4, 8],
+            6: [1, 2, 4, 8], 7: [1, 2, 4, 6], 8: [1, 2, 4], 9: [1, 2, 3], 10: [1, 2],
+            11: [1, 2], 12: [1]}
+
+
+def edge_shapes():
+    """Every routine with a full row, a short row, and (largest S) ragged rows."""
+    out = []
+    for n, opts in ROUTINES.items():
+        prev = 0
+        for k, S in enumerate(opts):
+            out.append((n, S, 1 + k % 2, 1 + (k + 1) % 3))
+            if S - prev > 1:
+                out.append((n, prev + 1, 2, 1))
+            prev = S
+        out.append((n, 2 * opts[-1] + 1, 2, 2))
+        out.append((n, 3 * opts[-1] - 1, 1, 1))
+    return out
+
+
+EDGE_SHAPES = edge_shapes()
 
 
 @pytest.mark.parametrize("kind", [fa.FILL_INT, fa.FILL_RAND])
@@ -73,7 +91,7 @@ def test_batch_edge_shapes_bitexact(torch_cuda, kind):
     b.close()
 
 
-@pytest.mark.parametrize("seg,bpc", [(1, 1), (2, 2), (3, 8), (4, 1), (4, 32)])
+@pytest.mark.parametrize("seg,bpc", [(1, 1), (2, 2), (3, 8), (4, 1), (8, 6), (64, 32)])
 def test_batch_tuning_bitexact(torch_cuda, seg, bpc):
     print("case batch tuning", seg, bpc, flush=True)
     shapes = mixed_shapes(40, seed=7, ns=(2, 3, 4, 5, 6, 7, 8), lo=1, hi=9)
@@ -144,7 +162,7 @@ def test_batch_tuner(torch_cuda):
     b = Batch(shapes)
     b.set_fields(fields)
     res = b.tune(reps=2)
-    assert res["seg_len"] in (1, 2, 3, 4) and res["blocks_per_cu"] in (1, 2, 3, 4, 8)
+    assert res["seg_len"] in (2, 4, 8, 16, 64) and res["blocks_per_cu"] in (1, 2, 3, 4, 6, 8)
     assert res["us_per_step"] > 0
     # the tuner stepped the fields; restore them and check one step
     for d, k in zip(fields, keep):
