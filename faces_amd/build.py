@@ -1,11 +1,10 @@
 """Builds the native parts of faces_amd in-tree.
 
   faces_amd/lib/libfaces_amd.so   HIP kernels + C ABI (include/faces_amd.h), gfx950
-  faces_amd/lib/faces_run         C++ stdin driver over the C ABI
 
 Run ``python faces_amd/build.py`` (or ``__graft_entry__.build()``).  The
-commands are plain hipcc/g++ invocations; every HIP translation unit is built
-for gfx950 only and without packed-FP32 instructions.
+command is one plain hipcc invocation; every HIP translation unit is built for
+gfx950 only and without packed-FP32 instructions.
 """
 from __future__ import annotations
 
@@ -65,24 +64,8 @@ def build_library(verbose: bool = True) -> str:
     return out
 
 
-def build_driver() -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
-    out = os.path.join(LIBDIR, "faces_run")
-    src = os.path.join(CSRC, "faces_run.cpp")
-    cmd = [
-        "g++", "-std=c++17", "-O2", "-Wall", f"-I{INCLUDE}", "-D__HIP_PLATFORM_AMD__",
-        "-I/opt/rocm/include", src, "-o", out,
-        f"-L{LIBDIR}", "-lfaces_amd", "-Wl,-rpath,$ORIGIN",
-        "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib",
-    ]
-    _run(cmd)
-    return out
-
-
 def build_all() -> None:
     build_library()
-    if os.path.exists(os.path.join(CSRC, "faces_run.cpp")):
-        build_driver()
 
 
 if __name__ == "__main__":
