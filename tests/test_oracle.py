@@ -308,6 +308,15 @@ def test_compare_counts_and_reference_metric():
     v[5] = np.nextafter(v[5], 2.0)
     r = oracle.compare(u, v, *shape)
     assert r["ndiff"] == 1 and r["max_ulp"] == 1 and r["argmax"] == 5
+    # gpu/main.cc:55-66: |u-v|/(|u|+|v|), skipped when the denominator is <= 1e-15
+    w = u.copy()
+    w[7] = -w[7]
+    r = oracle.compare(u, w, *shape)
+    assert r["max_rel"] == 1.0 and not r["pass"] and r["argmax"] == 7
+    z, z2 = np.zeros_like(u), np.zeros_like(u)
+    z2[3] = 1e-16
+    r = oracle.compare(z, z2, *shape)
+    assert r["max_rel"] == 0.0 and r["pass"] and r["ndiff"] == 1
 
 
 # ------------------------------------ a planted mistake is caught (in the oracle output)
@@ -327,3 +336,16 @@ def test_planted_mistakes_are_detected():
     bad2 = u.copy()
     oracle.share_sweep(bad2, *shape, periodic=False)
     assert not np.array_equal(bad2, want)
+
+
+def test_planted_transposed_index_is_detected():
+    """mx and my exchanged (a transposed element grid) on a non-cubic grid."""
+    shape = (3, 4, 2, 3)
+    u = oracle.new_field(*shape, oracle.FILL_INT, seed=13)
+    want = brute.share(brute.as6(u, *shape), *shape, periodic=True).ravel()
+    bad = u.copy()
+    oracle.share_sweep(bad, shape[0], shape[2], shape[1], shape[3], periodic=True)
+    assert not np.array_equal(bad, want)
+    good = u.copy()
+    oracle.share_sweep(good, *shape, periodic=True)
+    assert np.array_equal(good, want)
