@@ -62,7 +62,8 @@ struct faces_plan {
   int variant;   // FACES_VARIANT_*
   int lds_block; // threads per block of the LDS-staged kernel
   int stream_bpc; // persistent blocks per CU of the streaming kernel
-  int stream_walk;   // streaming kernel traversal: 0 items, 1 x walk, 2 y walk
+  int stream_walk;   // streaming kernel traversal: 0 items, 1 x walk, 2 y walk, 3 row groups
+  int stream_rows;   // walk 3: rows per group (the largest divisor of my not above it)
   // message arena: send slots then receive slots
   double *arena;
   long arena_doubles;
@@ -247,7 +248,6 @@ bool stream_ok(const faces_plan *p)
 template <int N, int S>
 int launch_stream_ns(const faces_plan *p, const Geom &G, hipStream_t st)
 {
-  using SH = StreamShape<N, S>;
   StreamItems I;
   I.my = p->my;
   I.mz = p->mz;
@@ -256,13 +256,24 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, hipStream_t st)
   I.field_bytes = (unsigned)(8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps));
   I.xcd_remap = p->xcd_remap;
   I.walk = p->stream_walk;
-  const long units = (I.walk == 1) ? (long)p->my * p->mz : (I.walk == 2) ? (long)p->mz * I.nseg
-                                                                         : I.nitems;
+  I.rows = 1;
+  if (I.walk == 3)
+    for (int r = std::max(1, std::min(p->stream_rows, p->my)); r >= 1; r--)
+      if (p->my % r == 0) {
+        I.rows = r;
+        break;
+      }
+  const long units = (I.walk == 1)   ? (long)p->my * p->mz
+                     : (I.walk == 2) ? (long)p->mz * I.nseg
+                     : (I.walk == 3) ? (long)p->mz * (p->my / I.rows)
+                                     : I.nitems;
   // stream_bpc = 0: one unit per block (no persistence)
   long nb = p->stream_bpc > 0 ? (long)device_cus() * p->stream_bpc : units;
   if (nb > units) nb = units;
   if (I.xcd_remap) nb = (nb + 7) / 8 * 8;
-  k_stream<N, S><<<dim3((unsigned)nb), dim3(kStreamThreads), SH::P * sizeof(double), st>>>(G, I);
+  const size_t lds = stream_lds_doubles<N, S>(I.walk, I.rows) * sizeof(double);
+  if (lds > 64 * 1024) return fail(FACES_ERR_ARG, "streaming kernel LDS %zu B above 64 KiB", lds);
+  k_stream<N, S><<<dim3((unsigned)nb), dim3(kStreamThreads), lds, st>>>(G, I);
   FACES_HIP(hipGetLastError());
   return FACES_OK;
 }
@@ -473,6 +484,7 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
   p->lds_block = 256;
   p->stream_bpc = 2;   // measured best on 60^3 n=7 and 32^3 n=8
   p->stream_walk = 1;
+  p->stream_rows = 4;
 
   // Message sizes, gpu/v-26-messages/Faces.cc:44-47, 64-66.
   const long nn = (long)n * n;
@@ -594,8 +606,12 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       p->stream_bpc = value;
       return FACES_OK;
     case FACES_OPT_STREAM_WALK:
-      if (value < 0 || value > 2) return fail(FACES_ERR_ARG, "walk %d outside [0,2]", value);
+      if (value < 0 || value > 3) return fail(FACES_ERR_ARG, "walk %d outside [0,3]", value);
       p->stream_walk = value;
+      return FACES_OK;
+    case FACES_OPT_STREAM_ROWS:
+      if (value < 1 || value > 64) return fail(FACES_ERR_ARG, "rows per group %d outside [1,64]", value);
+      p->stream_rows = value;
       return FACES_OK;
     default: return fail(FACES_ERR_ARG, "unknown option %d", option);
   }
@@ -612,6 +628,7 @@ int faces_plan_get_option(const faces_plan *p, int option, int *value)
     case FACES_OPT_LDS_BLOCK: *value = p->lds_block; break;
     case FACES_OPT_STREAM_BPC: *value = p->stream_bpc; break;
     case FACES_OPT_STREAM_WALK: *value = p->stream_walk; break;
+    case FACES_OPT_STREAM_ROWS: *value = p->stream_rows; break;
     case FACES_OPT_ACTIVE_KERNEL:
       *value = (p->variant == FACES_VARIANT_DIRECT) ? FACES_VARIANT_DIRECT
                : (p->variant == FACES_VARIANT_LDS && elem_lds_ok(p)) ? FACES_VARIANT_LDS

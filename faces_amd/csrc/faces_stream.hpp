@@ -59,8 +59,19 @@ struct StreamItems {
   int nitems;           // my*mz*nseg
   unsigned field_bytes; // below 2^32 - 8: offsets are 32-bit
   int xcd_remap;        // 1: contiguous ranges per XCD
-  int walk;             // 0 items, 1 rows in x (seam carried), 2 columns in y (y rows carried)
+  int walk;             // 0 items, 1 rows in x (seam carried), 2 columns in y (y rows carried),
+                        // 3 groups of `rows` rows, segment by segment (seam and y rows carried)
+  int rows;             // walk 3: rows per group (divides my)
 };
+
+// LDS doubles of the streaming kernel: the item footprint plus, for the walks
+// that carry the seam, one saved seam column per row of the group.
+template <int N, int S>
+__host__ __device__ constexpr int stream_lds_doubles(int walk, int rows)
+{
+  return StreamShape<N, S>::P +
+         ((walk == 1) ? 1 : (walk == 3) ? rows : 0) * StreamShape<N, S>::cS;
+}
 
 template <int N, int S>
 __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I)
@@ -76,10 +87,14 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
   unsigned voff[J];
   unsigned keepm = 0;
   unsigned rowsel[SH::JP];  // PART slots: row 0 own, 1 (jy-1), 2 (jz-1), 3 both; bit 2 = seam
-  // Walks carry part of the footprint from the previous item's LDS instead of
-  // reloading it: the seam column (x walk) or the y-partner rows (y walk).
-  unsigned carrym = 0;         // PART slots whose value is carried
-  unsigned carrysrc[SH::JP];   // their LDS position in the previous item
+  // Walks carry part of the footprint instead of reloading it: the y-partner
+  // rows come from the previous item's LDS (y walks), the seam column from a
+  // copy saved in LDS by the same row's previous segment (x walks).
+  const bool zig = I.walk == 1 || I.walk == 3;  // segment by segment along rows
+  const int R = (I.walk == 3) ? I.rows : 1;     // rows per group
+  unsigned carryY = 0, carryS = 0;  // PART slots carried: y-partner rows / seam
+  unsigned carrysrc[SH::JP];        // LDS position, in the source item, of a carried value
+  unsigned ringq[SH::JP];           // seam index (0..n^2) of a seam slot
 #pragma unroll
   for (int j = 0; j < JO; j++) {
     const int q = j * kStreamThreads + tid;
@@ -134,21 +149,22 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     rowsel[jp] = rs;
     if (keep) keepm |= 1u << j;
     carrysrc[jp] = 0;
-    if (I.walk == 1 && (rs & 4)) {
+    ringq[jp] = (rs & 4) ? (unsigned)q : 0u;
+    if (zig && (rs & 4)) {
       // x walk: the seam is the previous segment's last element's ix = n-1
       // column, which that item never changes.
-      carrym |= 1u << jp;
+      carryS |= 1u << jp;
       if (q < K * K) carrysrc[jp] = (S - 1) * (K * n2) + (q / K) * n2 + (q % K) * n + K;
       else if (q < K * K + K) carrysrc[jp] = SH::rY + (S - 1) * (K * n) + (q - K * K) * n + K;
       else if (q < K * K + 2 * K) carrysrc[jp] = SH::rZ + (S - 1) * (K * n) + (q - K * K - K) * n + K;
       else carrysrc[jp] = SH::rW + (S - 1) * n + K;
-    } else if (I.walk == 2 && rs == 1) {
+    } else if ((I.walk == 2 || I.walk == 3) && rs == 1) {
       // y walk: the y-partner rows are the previous row's x-rows iy = n-1,
       // loaded (never changed) by that item as part of its OWN region.
       const int qy = jp * kStreamThreads + tid;
       const int e = qy / (K * n), rr = qy - e * (K * n);
       const int iz = rr / n, ix = rr - (rr / n) * n;
-      carrym |= 1u << jp;
+      carryY |= 1u << jp;
       carrysrc[jp] = e * (K * n2) + iz * n2 + K * n + ix;
     }
   }
@@ -198,10 +214,15 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
 
   // ---- traversal.  Work units are dealt to blocks in XCD-contiguous ranges
   // (block b runs on XCD b % 8) or in one global order.  A unit is one item
-  // (walk 0), an element row walked in x (walk 1), or a column of rows at
-  // fixed (jz, segment) walked in y (walk 2).  Items that share memory
-  // sectors then run back to back on one CU, whose L2 merges their writes.
-  const int units = (I.walk == 1) ? I.my * I.mz : (I.walk == 2) ? I.mz * I.nseg : I.nitems;
+  // (walk 0), an element row walked in x (walk 1), a column of rows at
+  // fixed (jz, segment) walked in y (walk 2), or a group of R rows walked
+  // segment by segment, the R rows of a segment in turn (walk 3).  Items that
+  // share memory sectors then run back to back on one CU, whose L2 merges
+  // their writes, and carried values are not fetched twice.
+  const int units = (I.walk == 1) ? I.my * I.mz
+                    : (I.walk == 2) ? I.mz * I.nseg
+                    : (I.walk == 3) ? I.mz * (I.my / R)
+                                    : I.nitems;
   int start, cnt, k, nbx;
   if (I.xcd_remap) {
     const int xcd = blockIdx.x & 7;
@@ -217,14 +238,15 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     nbx = gridDim.x;
   }
   if (k >= cnt) return;
-  const int nsegu = (I.walk == 1) ? I.nseg : (I.walk == 2) ? G.my : 1;  // items per unit
+  const int nsegu = zig ? R * I.nseg : (I.walk == 2) ? G.my : 1;  // items per unit
   int sg = 0;
   auto item_of = [&](int kk, int ss) {
     const int c = start + kk;
-    if (I.walk == 1) return c * I.nseg + ss;
+    if (zig) return (c * R + ss % R) * I.nseg + ss / R;  // row c*R + r, segment ss/R
     if (I.walk == 2) return ((c / I.nseg) * G.my + ss) * I.nseg + (c % I.nseg);
     return c;
   };
+  double *const ring = lds + SH::P;  // saved seam columns, one per row of the group
 
   // Byte offsets into the field through one buffer resource: a store whose
   // offset is out of range is dropped by the hardware, so the stores of
@@ -263,21 +285,22 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
   };
   typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
   double v[J];
-  // carry: a carried slot comes from the previous item's LDS, so its load is
-  // sent out of range (no memory access, the instruction count is unchanged)
-  auto load_item = [&](bool carry) {
+  // A carried slot is not loaded: its load is sent out of range (no memory
+  // access, the instruction count is unchanged).
+  auto load_item = [&](bool cy, bool cs) {
+    const unsigned cm = (cy ? carryY : 0u) | (cs ? carryS : 0u);
 #pragma unroll
     for (int j = 0; j < J; j++) {
       unsigned off = ((j < JO) ? obase : pbase[j - JO < 0 ? 0 : j - JO]) + voff[j];
-      if (j >= JO && carry && (carrym & (1u << (j - JO)))) off = kOOB;
+      if (j >= JO && (cm & (1u << (j - JO)))) off = kOOB;
       v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
     }
   };
 
   int it = item_of(k, 0);
   bases(it);
-  load_item(false);
-  bool carry = false;
+  load_item(false, false);
+  bool cy = false, cs = false;  // the current item's y rows / seam are carried
   double vcar[SH::JP];
 #pragma unroll
   for (int jp = 0; jp < SH::JP; jp++) vcar[jp] = 0.0;
@@ -289,10 +312,15 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, 0.0), rsrc, kOOB - 8 * j, 0, 0);
   while (true) {
     __syncthreads();  // the previous item's stores have read LDS
+    const int rr = sg % R;  // the current item's row within the group
 #pragma unroll
     for (int j = 0; j < J; j++) {
       double x = v[j];
-      if (j >= JO && carry && (carrym & (1u << (j - JO)))) x = vcar[j - JO < 0 ? 0 : j - JO];
+      if (j >= JO) {
+        const int jp = j - JO < 0 ? 0 : j - JO;
+        if (cy && (carryY & (1u << jp))) x = vcar[jp];
+        if (cs && (carryS & (1u << jp))) x = ring[rr * SH::cS + ringq[jp]];
+      }
       lds[j * kStreamThreads + tid] = x;
     }
     // this item's store offsets, then the next item's loads (in flight
@@ -307,17 +335,24 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
       nk = k + nbx;
     }
     const bool more = nk < cnt;
-    const bool ncarry = more && ns > 0;
+    // the next item's carried parts (only within one unit)
+    const bool ncy = more && ns > 0 && (I.walk == 2 || (I.walk == 3 && ns % R != 0));
+    const bool ncs = more && zig && ns >= R;
     // Unconditional loads keep the wait accounting exact; after the last item
     // they re-read its own locations (in bounds, never used).
     const int nit = more ? item_of(nk, ns) : it;
     bases(nit);
-    load_item(ncarry);
+    load_item(ncy, ncs);
     __syncthreads();
-    if (ncarry) {
+    if (ncy) {
 #pragma unroll
       for (int jp = 0; jp < SH::JP; jp++)
-        if (carrym & (1u << jp)) vcar[jp] = lds[carrysrc[jp]];
+        if (carryY & (1u << jp)) vcar[jp] = lds[carrysrc[jp]];
+    }
+    if (zig && sg + R < nsegu) {  // the row's next segment takes this seam
+#pragma unroll
+      for (int jp = 0; jp < SH::JP; jp++)
+        if (carryS & (1u << jp)) ring[rr * SH::cS + ringq[jp]] = lds[carrysrc[jp]];
     }
     // shared nodes: x pairs, then y, then z
 #pragma unroll
@@ -363,7 +398,8 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     k = nk;
     sg = ns;
     it = nit;
-    carry = ncarry;
+    cy = ncy;
+    cs = ncs;
   }
 }
 

@@ -117,8 +117,11 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *   FACES_OPT_STREAM_WALK    STREAM kernel traversal: 0 = independent items,
  *                            1 = a block walks element rows in x carrying the
  *                            seam column, 2 = a block walks columns of rows in
- *                            y carrying the y-partner rows
- *   FACES_OPT_ACTIVE_KERNEL  (get only) the variant faces_share launches */
+ *                            y carrying the y-partner rows, 3 = a block walks
+ *                            a group of rows segment by segment, carrying both
+ *   FACES_OPT_ACTIVE_KERNEL  (get only) the variant faces_share launches
+ *   FACES_OPT_STREAM_ROWS    STREAM walk 3: rows per group (the largest divisor
+ *                            of my not above the value is used) */
 #define FACES_OPT_VARIANT 0
 #define FACES_OPT_DIRECT_BLOCK 1
 #define FACES_OPT_SEG 2
@@ -127,6 +130,7 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
 #define FACES_OPT_STREAM_BPC 5
 #define FACES_OPT_STREAM_WALK 6
 #define FACES_OPT_ACTIVE_KERNEL 7
+#define FACES_OPT_STREAM_ROWS 8
 int faces_plan_set_option(faces_plan *plan, int option, int value);
 int faces_plan_get_option(const faces_plan *plan, int option, int *value);
 

@@ -97,17 +97,19 @@ def test_periodic_variants_bitexact(torch_cuda, shape, variant):
     assert_same(run_gpu(torch_cuda, u, shape, variant=variant), run_oracle(u, shape), shape)
 
 
-@pytest.mark.parametrize("walk", [0, 1, 2])
+@pytest.mark.parametrize("walk,rows", [(0, 1), (1, 1), (2, 1), (3, 1), (3, 2), (3, 3), (3, 4)])
 @pytest.mark.parametrize("seg,bpc,xcd", [(1, 0, 1), (2, 1, 0), (3, 2, 1), (4, 3, 1)])
-@pytest.mark.parametrize("shape", [(7, 12, 5, 3), (8, 4, 3, 5), (5, 6, 1, 2), (4, 3, 4, 1),
+@pytest.mark.parametrize("shape", [(7, 12, 6, 3), (8, 4, 3, 5), (5, 6, 1, 2), (4, 3, 4, 1),
                                    (6, 2, 2, 2)])
-def test_stream_walks_bitexact(torch_cuda, walk, seg, bpc, xcd, shape):
+def test_stream_walks_bitexact(torch_cuda, walk, rows, seg, bpc, xcd, shape):
     n, mx, my, mz = shape
+    print("case stream walk", walk, rows, seg, bpc, xcd, shape, flush=True)
     u = oracle.new_field(*shape, fa.FILL_RAND, seed=303)
     d = to_dev(torch_cuda, u)
     f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
     f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
     f.set_option(fa.OPT_STREAM_WALK, walk)
+    f.set_option(fa.OPT_STREAM_ROWS, rows)
     f.set_option(fa.OPT_SEG, seg)
     f.set_option(fa.OPT_STREAM_BPC, bpc)
     f.set_option(fa.OPT_XCD, xcd)
