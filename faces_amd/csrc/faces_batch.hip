@@ -23,7 +23,8 @@ namespace faces {
 // that touch the same sectors (neighbouring segments, partner rows) are then
 // processed at about the same time under one L2, where partial-sector writes
 // of different blocks merge before they reach HBM.  The item's grid is found
-// by walking forward from the previous item's grid.
+// by walking forward from the previous item's grid; the block's consecutive
+// items of one grid form a run, summed with the next item's loads in flight.
 
 // (n, S) routines: per n, S from 1 up to about 2K points per item.
 #define FACES_BATCH_CASES(X)                                                                  \
@@ -35,30 +36,59 @@ namespace faces {
 // One out-of-line routine per (n, S): inlining all of them into the kernel's
 // switch makes the register allocator size every path for the union of their
 // live ranges (256 VGPRs and spills); as calls, each keeps its own allocation.
-template <int N, int S>
-__device__ __noinline__ void batch_item(const BatchGrid G, int it, double *lds)
+// Arguments of a call arrive in vector registers; they are block-uniform, so
+// they are moved back to scalar registers before use.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ long uni(long x)
 {
-  stream_item<N, S>(G, it, lds);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(x & 0xffffffffu));
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long)x >> 32));
+  return (long)(((unsigned long)hi << 32) | lo);
 }
 
-__global__ __launch_bounds__(kStreamThreads) void k_batch(BatchLaunch D)
+template <int N, int S>
+__device__ __noinline__ void batch_run(const BatchGrid G0, int it, int stride, int count, double *lds)
+{
+  BatchGrid G;
+  G.u = (double *)uni((long)G0.u);
+  G.ys = uni(G0.ys);
+  G.ps = uni(G0.ps);
+  G.field_bytes = (unsigned)uni((int)G0.field_bytes);
+  G.n = uni(G0.n);
+  G.mx = uni(G0.mx);
+  G.my = uni(G0.my);
+  G.mz = uni(G0.mz);
+  G.S = uni(G0.S);
+  G.nseg = uni(G0.nseg);
+  G.items = uni(G0.items);
+  G.first = uni(G0.first);
+  stream_run<N, S>(G, uni(it), uni(stride), uni(count), lds);
+}
+
+// Four blocks per CU: the callees are compiled within the matching register
+// budget (measured faster than letting them grow and run two blocks per CU).
+__global__ __launch_bounds__(kStreamThreads, 4) void k_batch(BatchLaunch D)
 {
   extern __shared__ double lds[];
   const int x = blockIdx.x % kXcds;
   const int stride = gridDim.x / kXcds;
   const int end = D.xr[x + 1];
   int g = 0;
-  for (int u = D.xr[x] + (int)(blockIdx.x / kXcds); u < end; u += stride) {
+  for (int u = D.xr[x] + (int)(blockIdx.x / kXcds); u < end;) {
     while (g + 1 < D.ngrids && D.grids[g + 1].first <= u) g++;
     const BatchGrid G = D.grids[g];
+    // the block's items in this grid: u, u + stride, ... below the grid's end
+    const int last = min(end, G.first + G.items);
+    const int count = (last - u + stride - 1) / stride;
     const int it = u - G.first;
     switch (G.n * 64 + G.S) {
 #define FACES_BATCH_CASE(NN, SS) \
-  case NN * 64 + SS: batch_item<NN, SS>(G, it, lds); break;
+  case NN * 64 + SS: batch_run<NN, SS>(G, it, stride, count, lds); break;
       FACES_BATCH_CASES(FACES_BATCH_CASE)
 #undef FACES_BATCH_CASE
       default: break;  // excluded on the host
     }
+    u += count * stride;
   }
 }
 

@@ -29,16 +29,18 @@ struct BatchGrid {
   int first;             // index of the grid's first item in the work list
 };
 
+// Offsets of one item's footprint slots for the calling thread: load offset
+// (kOOB for absent elements) and store offset (kOOB for positions the item
+// must not store).  ecnt: elements present in the item.
 template <int N, int S>
-__device__ __forceinline__ void stream_item(const BatchGrid &G, int it, double *lds)
+__device__ __forceinline__ int item_offsets(const BatchGrid &G, int it, unsigned (&ld)[StreamShape<N, S>::J],
+                                            unsigned (&st)[StreamShape<N, S>::J])
 {
   using SH = StreamShape<N, S>;
-  constexpr int n = N, K = N - 1, n2 = N * N, J = SH::J, JO = SH::JO, NN = SH::NN;
+  constexpr int n = N, K = N - 1, n2 = N * N, JO = SH::JO;
   constexpr unsigned n3 = (unsigned)(N * N * N);
   constexpr unsigned kOOB = 0xFFFFFFF8u;
-  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
   const int tid = threadIdx.x;
-
   const int sg = it % G.nseg;
   const int row = it / G.nseg;
   const int jy = row % G.my, jz = row / G.my;
@@ -49,11 +51,6 @@ __device__ __forceinline__ void stream_item(const BatchGrid &G, int it, double *
   const int ecnt = min(S, G.mx - jx0);  // elements present in this item
   const unsigned r[4] = {(unsigned)((jz * G.ps + jy * G.ys) * 8), (unsigned)((jz * G.ps + jym * G.ys) * 8),
                          (unsigned)((jzm * G.ps + jy * G.ys) * 8), (unsigned)((jzm * G.ps + jym * G.ys) * 8)};
-  const __amdgpu_buffer_rsrc_t rsrc =
-      __builtin_amdgcn_make_buffer_rsrc(G.u, 0, (int)G.field_bytes, 0x00020000);
-
-  // footprint slot -> byte offset (kOOB for positions the item must not store)
-  unsigned ld[J], st[J];
 #pragma unroll
   for (int j = 0; j < JO; j++) {
     const int q = j * kStreamThreads + tid;
@@ -97,16 +94,17 @@ __device__ __forceinline__ void stream_item(const BatchGrid &G, int it, double *
     ld[j] = off;
     st[j] = keep ? off : kOOB;
   }
+  return ecnt;
+}
 
-  double v[J];
-#pragma unroll
-  for (int j = 0; j < J; j++)
-    v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, ld[j], 0, 0));
-#pragma unroll
-  for (int j = 0; j < J; j++) lds[j * kStreamThreads + tid] = v[j];
-  __syncthreads();
-
-  // shared nodes of the item's elements: x pairs, then y, then z
+// The shared nodes of an item's first ecnt elements, summed in LDS: x pairs,
+// then y, then z.
+template <int N, int S>
+__device__ __forceinline__ void item_sums(double *lds, int ecnt)
+{
+  using SH = StreamShape<N, S>;
+  constexpr int n = N, K = N - 1, n2 = N * N, NN = SH::NN;
+  const int tid = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < NN; i++) {
     const int k = i * kStreamThreads + tid;
@@ -158,12 +156,63 @@ __device__ __forceinline__ void stream_item(const BatchGrid &G, int it, double *
       }
     }
   }
-  __syncthreads();
+}
+
+// A run of `count` items of one grid, it0, it0 + stride, ...: item t is
+// summed while the loads of item t+1 are in flight.  Operations are issued
+// per item as J loads (one item ahead), then J stores; J dropped stores
+// before the loop enter it in that same state, so the waits are exact.
+template <int N, int S>
+__device__ __forceinline__ void stream_run(const BatchGrid &G, int it0, int stride, int count, double *lds)
+{
+  using SH = StreamShape<N, S>;
+  constexpr int J = SH::J;
+  constexpr unsigned kOOB = 0xFFFFFFF8u;
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  const int tid = threadIdx.x;
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(G.u, 0, (int)G.field_bytes, 0x00020000);
+  double v[J];
+  {
+    unsigned ld[J], st[J];
+    item_offsets<N, S>(G, it0, ld, st);
+#pragma unroll
+    for (int j = 0; j < J; j++)
+      v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, ld[j], 0, 0));
+  }
 #pragma unroll
   for (int j = 0; j < J; j++)
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, lds[j * kStreamThreads + tid]),
-                                          rsrc, st[j], 0, 0);
-  __syncthreads();  // LDS is reused by the block's next item
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, 0.0), rsrc, kOOB - 8 * j, 0, 0);
+  int it = it0;
+  for (int t = 0; t < count; t++) {
+    __syncthreads();  // the previous item's stores (or the previous run) have read LDS
+#pragma unroll
+    for (int j = 0; j < J; j++) lds[j * kStreamThreads + tid] = v[j];
+    // the next item's loads (the last item re-reads its own locations: in
+    // bounds, never used); offsets are recomputed where they are used, which
+    // keeps only the loaded values live across the item
+    const int nit = (t + 1 < count) ? it + stride : it;
+    {
+      unsigned ld[J], st[J];
+      item_offsets<N, S>(G, nit, ld, st);
+#pragma unroll
+      for (int j = 0; j < J; j++)
+        v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, ld[j], 0, 0));
+    }
+    __syncthreads();
+    item_sums<N, S>(lds, min(S, G.mx - (it % G.nseg) * S));
+    __syncthreads();
+    {
+      unsigned ld[J], st[J];
+      item_offsets<N, S>(G, it, ld, st);
+#pragma unroll
+      for (int j = 0; j < J; j++)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, lds[j * kStreamThreads + tid]),
+                                              rsrc, st[j], 0, 0);
+    }
+    it = nit;
+  }
+  __syncthreads();  // LDS is reused by the block's next run
 }
 
 }  // namespace faces
