@@ -31,7 +31,7 @@ HEADER = os.path.join(ROOT, "include", "faces_amd.h")
 
 OPEN, PERIODIC = 0, 1
 PATH_AUTO, PATH_FUSED, PATH_STAGED, PATH_STAGED_DIRECT = 0, 1, 2, 3
-VARIANT_AUTO, VARIANT_DIRECT, VARIANT_LDS, VARIANT_STREAM = 0, 1, 2, 3
+VARIANT_AUTO, VARIANT_DIRECT, VARIANT_LDS, VARIANT_STREAM, VARIANT_ITEMS = 0, 1, 2, 3, 4
 (OPT_VARIANT, OPT_DIRECT_BLOCK, OPT_SEG, OPT_XCD, OPT_LDS_BLOCK, OPT_STREAM_BPC,
  OPT_STREAM_WALK, OPT_ACTIVE_KERNEL, OPT_STREAM_ROWS) = range(9)
 FILL_INT, FILL_RAND, FILL_REF = 0, 1, 2

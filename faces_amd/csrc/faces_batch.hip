@@ -17,14 +17,13 @@ namespace faces {
 // ---------------------------------------------------------------------------
 // the batched kernel
 // ---------------------------------------------------------------------------
-// Static schedule, no atomics: the work list is cut into one range per XCD
-// (equal estimated cost), and the blocks that the hardware places on XCD x
-// (blockIdx % 8 == x, round-robin dispatch) stride through range x.  Items
-// that touch the same sectors (neighbouring segments, partner rows) are then
-// processed at about the same time under one L2, where partial-sector writes
-// of different blocks merge before they reach HBM.  The item's grid is found
-// by walking forward from the previous item's grid; the block's consecutive
-// items of one grid form a run, summed with the next item's loads in flight.
+// Static schedule, no atomics: the work list is cut into one contiguous
+// chunk of equal estimated cost per block, the chunks of XCD x (blocks with
+// blockIdx % 8 == x, round-robin dispatch) forming a contiguous eighth of the
+// list.  A block's items of one grid are consecutive element segments and
+// rows: they form a run, summed with the next item's loads in flight, and the
+// neighbouring items whose sectors an item shares were summed just before by
+// the same block, so those lines are still in its L2.
 
 // (n, S) routines: per n, S from 1 up to about 2K points per item.
 #define FACES_BATCH_CASES(X)                                                                  \
@@ -70,25 +69,48 @@ __device__ __noinline__ void batch_run(const BatchGrid G0, int it, int stride, i
 __global__ __launch_bounds__(kStreamThreads, 4) void k_batch(BatchLaunch D)
 {
   extern __shared__ double lds[];
-  const int x = blockIdx.x % kXcds;
-  const int stride = gridDim.x / kXcds;
-  const int end = D.xr[x + 1];
-  int g = 0;
-  for (int u = D.xr[x] + (int)(blockIdx.x / kXcds); u < end;) {
-    while (g + 1 < D.ngrids && D.grids[g + 1].first <= u) g++;
-    const BatchGrid G = D.grids[g];
-    // the block's items in this grid: u, u + stride, ... below the grid's end
+  const int nb = gridDim.x, nbx = nb / kXcds;
+  const int c = (blockIdx.x % kXcds) * nbx + blockIdx.x / kXcds;  // this block's chunk
+  // last grid whose first item is at or before u / whose cost prefix is at or
+  // before t (the grids are few: binary search over scalar loads)
+  auto grid_of_item = [&](int u) {
+    int lo = 0, hi = D.ngrids - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (D.grids[mid].first <= u) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+  };
+  auto item_at = [&](double t) {
+    int g = 0;
+    if (D.grids) {
+      int lo = 0, hi = D.ngrids - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (D.grids[mid].cost0 <= t) lo = mid; else hi = mid - 1;
+      }
+      g = lo;
+    }
+    const BatchGrid &G = D.grids ? D.grids[g] : D.g0;
+    const double k = (t - G.cost0) / G.icost;
+    return G.first + (k <= 0.0 ? 0 : k >= (double)G.items ? G.items : (int)k);
+  };
+  int u = (c == 0) ? 0 : item_at(D.total * c / nb);
+  const int end = (c == nb - 1) ? D.nunits : item_at(D.total * (c + 1) / nb);
+  int g = (D.grids && u < end) ? grid_of_item(u) : 0;
+  while (u < end) {
+    const BatchGrid G = D.grids ? D.grids[g] : D.g0;
     const int last = min(end, G.first + G.items);
-    const int count = (last - u + stride - 1) / stride;
     const int it = u - G.first;
     switch (G.n * 64 + G.S) {
 #define FACES_BATCH_CASE(NN, SS) \
-  case NN * 64 + SS: batch_run<NN, SS>(G, it, stride, count, lds); break;
+  case NN * 64 + SS: batch_run<NN, SS>(G, it, 1, last - u, lds); break;
       FACES_BATCH_CASES(FACES_BATCH_CASE)
 #undef FACES_BATCH_CASE
       default: break;  // excluded on the host
     }
-    u += count * stride;
+    u = last;
+    g++;
   }
 }
 
@@ -144,10 +166,37 @@ int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t s
 // points plus a fixed per-item latency share.
 static double item_cost(const BatchGrid &x)
 {
-  return (double)std::min(x.S, x.mx) * x.n * x.n * x.n + 512.0;
+  return (double)std::min(x.S, x.mx) * x.n * x.n * x.n + 320.0;
 }
 
-int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, int *xr)
+int launch_single(double *u, int n, int mx, int my, int mz, long plane_stride, int seg_cap,
+                  int blocks_per_cu, hipStream_t st)
+{
+  BatchLaunch D;
+  BatchGrid &x = D.g0;
+  x.u = u;
+  x.n = n;
+  x.mx = mx;
+  x.my = my;
+  x.mz = mz;
+  x.ys = (long)n * n * n * mx;
+  x.ps = plane_stride > 0 ? plane_stride : x.ys * my;
+  x.field_bytes = (unsigned)(8 * ((mz - 1) * x.ps + x.ys * my));
+  x.S = batch_seg(n, mx, seg_cap);
+  if (x.S == 0) return fail(FACES_ERR_ARG, "grid n=%d mx=%d has no item shape", n, mx);
+  x.nseg = (mx + x.S - 1) / x.S;
+  x.items = my * mz * x.nseg;
+  x.first = 0;
+  x.cost0 = 0.0;
+  x.icost = 1.0;
+  D.grids = nullptr;
+  D.ngrids = 1;
+  D.nunits = x.items;
+  D.total = (double)x.items;
+  return launch_batch(D, batch_blocks(x.items, blocks_per_cu), batch_lds_bytes(n, x.S), st);
+}
+
+int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, double *total)
 {
   // Items of expensive grids first: the work list then ends on cheap items.
   std::vector<int> order(g.size());
@@ -157,38 +206,25 @@ int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, 
     if (x.S == 0) return fail(FACES_ERR_ARG, "grid n=%d mx=%d has no batched item shape", x.n, x.mx);
     x.nseg = (x.mx + x.S - 1) / x.S;
     x.items = x.my * x.mz * x.nseg;
+    x.icost = item_cost(x);
   }
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    return (long)g[a].S * g[a].n * g[a].n * g[a].n > (long)g[b].S * g[b].n * g[b].n * g[b].n;
-  });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g[a].icost > g[b].icost; });
   std::vector<BatchGrid> sorted;
   int first = 0, lds = 0;
+  double cost = 0.0;
   for (int i : order) {
     BatchGrid x = g[i];
     x.first = first;
+    x.cost0 = cost;
     first += x.items;
+    cost += x.items * x.icost;
     lds = std::max(lds, batch_lds_bytes(x.n, x.S));
     sorted.push_back(x);
   }
   g.swap(sorted);
   *lds_bytes = lds;
   *nunits = first;
-  // XCD ranges of equal cost (cut at item granularity)
-  double total = 0;
-  for (const auto &x : g) total += x.items * item_cost(x);
-  xr[0] = 0;
-  int k = 1;
-  double acc = 0;
-  for (const auto &x : g) {
-    const double c = item_cost(x);
-    while (k < kXcds && acc + x.items * c >= total * k / kXcds) {
-      const long cut = x.first + (long)std::ceil((total * k / kXcds - acc) / c);
-      xr[k] = (int)std::min<long>(std::max<long>(cut, xr[k - 1]), x.first + x.items);
-      k++;
-    }
-    acc += x.items * c;
-  }
-  while (k <= kXcds) xr[k++] = first;
+  *total = cost;
   return FACES_OK;
 }
 
@@ -205,7 +241,7 @@ struct faces_batch {
   std::vector<int> solo;            // grids without a batched item shape (n > 12): own plan
   // device state
   BatchGrid *d_grids = nullptr;
-  int xr[kXcds + 1] = {};
+  double total = 0.0;
   int lds_bytes = 0, nunits = 0, nbatched = 0;
   // tuning
   int seg_cap = 64, bpc = 4;
@@ -243,7 +279,7 @@ int upload(faces_batch *b)
   b->lds_bytes = 0;
   b->nunits = 0;
   if (!g.empty()) {
-    int st = plan_grids(g, b->seg_cap, &b->lds_bytes, &b->nunits, b->xr);
+    int st = plan_grids(g, b->seg_cap, &b->lds_bytes, &b->nunits, &b->total);
     if (st) return st;
     // launches of the old descriptors may still run on any stream
     FACES_HIP(hipDeviceSynchronize());
@@ -264,11 +300,11 @@ int upload(faces_batch *b)
 
 BatchLaunch launch_desc(const faces_batch *b)
 {
-  BatchLaunch D;
+  BatchLaunch D = {};
   D.grids = b->d_grids;
   D.ngrids = b->nbatched;
   D.nunits = b->nunits;
-  std::copy(b->xr, b->xr + kXcds + 1, D.xr);
+  D.total = b->total;
   return D;
 }
 

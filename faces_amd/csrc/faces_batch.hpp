@@ -10,14 +10,17 @@
 
 namespace faces {
 
-// One batched launch: grids sorted by first item; items [xr[x], xr[x+1]) go
-// to the blocks that the hardware dispatches to XCD x (blocks x, x+8, ...).
+// One batched launch: grids sorted by first item.  The work list is cut into
+// one contiguous chunk of equal estimated cost per block; chunks are ordered
+// XCD by XCD (the hardware dispatches block b to XCD b % 8), so each XCD
+// holds a contiguous eighth of the list.
 constexpr int kXcds = 8;
 struct BatchLaunch {
-  const BatchGrid *grids;
+  const BatchGrid *grids;  // null: the single grid g0 (passed by value)
   int ngrids;
   int nunits;
-  int xr[kXcds + 1];
+  double total;            // estimated cost of the whole list
+  BatchGrid g0;
 };
 
 // Elements per item for a grid (largest S <= cap dividing mx with a kernel
@@ -25,15 +28,22 @@ struct BatchLaunch {
 int batch_seg(int n, int mx, int cap);
 int batch_lds_bytes(int n, int S);
 
-// Fills S/nseg/items, sorts the grids by item cost, assigns first-item
-// offsets and splits the work list into kXcds ranges of equal estimated
-// cost; returns the LDS bytes and item count of the launch.
-int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, int *xr);
+// Fills S/nseg/items, sorts the grids by item cost and assigns first-item
+// offsets and cost prefixes; returns the LDS bytes, item count and total
+// estimated cost of the launch.
+int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, double *total);
 
 // Persistent blocks for a launch: blocks_per_cu per CU, a multiple of kXcds.
 int batch_blocks(int nunits, int blocks_per_cu);
 
 // Enqueues the batched kernel (nblocks persistent blocks, a multiple of kXcds).
 int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t st);
+
+// One periodic grid (n <= 12, field below 4 GB) through the batched kernel:
+// element segments of up to seg_cap elements, the row's last one possibly
+// shorter.  The descriptor travels as a kernel argument: nothing is
+// allocated or copied, so the launch can be captured in a graph.
+int launch_single(double *u, int n, int mx, int my, int mz, long plane_stride, int seg_cap,
+                  int blocks_per_cu, hipStream_t st);
 
 }  // namespace faces

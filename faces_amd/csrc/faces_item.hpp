@@ -27,6 +27,7 @@ struct BatchGrid {
   int S, nseg;           // elements per item (the row's last item may hold fewer), items per row
   int items;             // my*mz*nseg
   int first;             // index of the grid's first item in the work list
+  double cost0, icost;   // estimated cost of the items before this grid; of one item
 };
 
 // Offsets of one item's footprint slots for the calling thread: load offset

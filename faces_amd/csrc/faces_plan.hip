@@ -9,6 +9,7 @@
 #include "faces_common.hpp"
 #include "faces_kernels.hpp"
 #include "faces_stream.hpp"
+#include "faces_batch.hpp"
 
 namespace faces {
 
@@ -64,6 +65,7 @@ struct faces_plan {
   int stream_bpc; // persistent blocks per CU of the streaming kernel
   int stream_walk;   // streaming kernel traversal: 0 items, 1 x walk, 2 y walk, 3 row groups
   int stream_rows;   // walk 3: rows per group (the largest divisor of my not above it)
+  int items_bpc;     // work-list item kernel: persistent blocks per CU
   // message arena: send slots then receive slots
   double *arena;
   long arena_doubles;
@@ -309,11 +311,37 @@ int launch_stream(const faces_plan *p, double *u, hipStream_t st)
   }
 }
 
+// The work-list item kernel (faces_batch.hip) takes ragged element rows: it
+// serves grids whose mx leaves the streaming kernel only 1- or 2-element
+// items.
+bool items_ok(const faces_plan *p)
+{
+  const long bytes = 8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps);
+  return p->n >= 2 && p->n <= 12 && bytes < 0xFFFFFFF0L && batch_seg(p->n, p->mx, 64) > 0;
+}
+
+// The sum kernel faces_share launches on a periodic field with compute = 1.
+int active_kernel(const faces_plan *p)
+{
+  switch (p->variant) {
+    case FACES_VARIANT_DIRECT: return FACES_VARIANT_DIRECT;
+    case FACES_VARIANT_LDS: return elem_lds_ok(p) ? FACES_VARIANT_LDS : FACES_VARIANT_DIRECT;
+    case FACES_VARIANT_STREAM: return stream_ok(p) ? FACES_VARIANT_STREAM : FACES_VARIANT_DIRECT;
+    case FACES_VARIANT_ITEMS: return items_ok(p) ? FACES_VARIANT_ITEMS : FACES_VARIANT_DIRECT;
+    default:
+      if (stream_ok(p) && stream_seg(p) >= 3) return FACES_VARIANT_STREAM;
+      if (items_ok(p)) return FACES_VARIANT_ITEMS;
+      return stream_ok(p) ? FACES_VARIANT_STREAM : FACES_VARIANT_DIRECT;
+  }
+}
+
 int launch_box(const faces_plan *p, double *u, bool skip_surface, hipStream_t st)
 {
-  if (!skip_surface && (p->variant == FACES_VARIANT_AUTO || p->variant == FACES_VARIANT_STREAM) &&
-      stream_ok(p))
-    return launch_stream(p, u, st);
+  const int kern = skip_surface ? FACES_VARIANT_DIRECT : active_kernel(p);
+  if (kern == FACES_VARIANT_STREAM) return launch_stream(p, u, st);
+  if (kern == FACES_VARIANT_ITEMS)
+    return launch_single(u, p->n, p->mx, p->my, p->mz, p->ps, p->seg_len > 0 ? p->seg_len : 64,
+                         p->items_bpc, st);
   if (!skip_surface && p->variant == FACES_VARIANT_LDS && elem_lds_ok(p))
     return launch_elem_lds(p, u, st);
   Geom G = make_geom(p, u);
@@ -485,6 +513,7 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
   p->stream_bpc = 2;   // measured best on 60^3 n=7 and 32^3 n=8
   p->stream_walk = 1;
   p->stream_rows = 4;
+  p->items_bpc = 4;
 
   // Message sizes, gpu/v-26-messages/Faces.cc:44-47, 64-66.
   const long nn = (long)n * n;
@@ -563,8 +592,16 @@ int faces_plan_set_tuning(faces_plan *p, int block, int seg_len, int xcd_remap)
 int faces_plan_set_variant(faces_plan *p, int variant, int block)
 {
   if (!p) return fail(FACES_ERR_ARG, "null plan");
-  if (variant < FACES_VARIANT_AUTO || variant > FACES_VARIANT_STREAM)
+  if (variant < FACES_VARIANT_AUTO || variant > FACES_VARIANT_ITEMS)
     return fail(FACES_ERR_ARG, "unknown kernel variant %d", variant);
+  if (variant == FACES_VARIANT_ITEMS) {
+    if (block < 0 || block > 4) return fail(FACES_ERR_ARG, "blocks per CU %d outside [0,4]", block);
+    if (!items_ok(p))
+      return fail(FACES_ERR_ARG, "item kernel does not fit this grid (n=%d, mx=%d)", p->n, p->mx);
+    p->variant = variant;
+    if (block) p->items_bpc = block;
+    return FACES_OK;
+  }
   if (variant == FACES_VARIANT_STREAM) {
     if (block < -1 || block > 32)
       return fail(FACES_ERR_ARG, "blocks per CU %d outside [-1,32] (-1: one item per block)", block);
@@ -629,11 +666,7 @@ int faces_plan_get_option(const faces_plan *p, int option, int *value)
     case FACES_OPT_STREAM_BPC: *value = p->stream_bpc; break;
     case FACES_OPT_STREAM_WALK: *value = p->stream_walk; break;
     case FACES_OPT_STREAM_ROWS: *value = p->stream_rows; break;
-    case FACES_OPT_ACTIVE_KERNEL:
-      *value = (p->variant == FACES_VARIANT_DIRECT) ? FACES_VARIANT_DIRECT
-               : (p->variant == FACES_VARIANT_LDS && elem_lds_ok(p)) ? FACES_VARIANT_LDS
-               : stream_ok(p) ? FACES_VARIANT_STREAM : FACES_VARIANT_DIRECT;
-      break;
+    case FACES_OPT_ACTIVE_KERNEL: *value = active_kernel(p); break;
     default: return fail(FACES_ERR_ARG, "unknown option %d", option);
   }
   return FACES_OK;

@@ -108,17 +108,17 @@ int faces_server::launch(Slot &s, std::vector<Request *> &batch)
   }
   if (!g.empty()) {
     int lds = 0, nunits = 0;
-    int xr[kXcds + 1];
-    int st = plan_grids(g, 64, &lds, &nunits, xr);
+    double total = 0.0;
+    int st = plan_grids(g, 64, &lds, &nunits, &total);
     if (st) return st;
     std::copy(g.begin(), g.end(), s.h_grids);
     FACES_HIP(hipMemcpyAsync(s.d_grids, s.h_grids, sizeof(BatchGrid) * g.size(), hipMemcpyHostToDevice,
                              s.stream));
-    BatchLaunch D;
+    BatchLaunch D = {};
     D.grids = s.d_grids;
     D.ngrids = (int)g.size();
     D.nunits = nunits;
-    std::copy(xr, xr + kXcds + 1, D.xr);
+    D.total = total;
     st = launch_batch(D, batch_blocks(nunits, 4), lds, s.stream);
     if (st) return st;
   }

@@ -95,14 +95,18 @@ int faces_plan_set_tuning(faces_plan *plan, int block, int seg_len, int xcd_rema
 /* Kernel of the in-place sum on periodic fields: FACES_VARIANT_DIRECT (one
  * lane per node, copies read straight from HBM), FACES_VARIANT_LDS (element
  * segments staged through LDS), FACES_VARIANT_STREAM (persistent streaming
- * kernel with item-invariant offset tables); AUTO picks STREAM when the
- * grid fits it.  block: threads per block of the LDS kernel (64..512,
- * multiple of 64), or persistent blocks per CU of the STREAM kernel (1..32,
- * or -1 for one work item per block); 0 keeps the current value. */
+ * kernel with item-invariant offset tables), FACES_VARIANT_ITEMS (the
+ * batched step's work-list kernel on this one grid: element segments of up
+ * to 32 elements, the row's last one shorter, so any mx works); AUTO picks
+ * STREAM when mx has a divisor of 3 or 4 elements per item, else ITEMS.
+ * block: threads per block of the LDS kernel (64..512, multiple of 64),
+ * persistent blocks per CU of the STREAM kernel (1..32, or -1 for one work
+ * item per block) or of the ITEMS kernel (1..4); 0 keeps the current value. */
 #define FACES_VARIANT_AUTO 0
 #define FACES_VARIANT_DIRECT 1
 #define FACES_VARIANT_LDS 2
 #define FACES_VARIANT_STREAM 3
+#define FACES_VARIANT_ITEMS 4
 int faces_plan_set_variant(faces_plan *plan, int variant, int block);
 
 /* Launch options of the in-place sum (no counterpart in the reference,

@@ -90,7 +90,8 @@ def test_periodic_fused_bitexact(torch_cuda, shape, kind):
 @pytest.mark.parametrize("variant", [(fa.VARIANT_DIRECT, 0), (fa.VARIANT_LDS, 64),
                                      (fa.VARIANT_LDS, 256), (fa.VARIANT_LDS, 512),
                                      (fa.VARIANT_STREAM, 1), (fa.VARIANT_STREAM, 8),
-                                     (fa.VARIANT_STREAM, -1)])
+                                     (fa.VARIANT_STREAM, -1), (fa.VARIANT_ITEMS, 1),
+                                     (fa.VARIANT_ITEMS, 4)])
 def test_periodic_variants_bitexact(torch_cuda, shape, variant):
     print("case periodic variant", shape, variant, flush=True)
     u = oracle.new_field(*shape, fa.FILL_RAND, seed=202)
@@ -303,3 +304,32 @@ def test_invalid_arguments(torch_cuda):
     with pytest.raises(fa.FacesError):
         f.set_tuning(100, 0, 0)
     f.close()
+
+
+@pytest.mark.parametrize("shape,kernel", [((7, 60, 2, 2), fa.VARIANT_STREAM),
+                                          ((7, 9, 2, 2), fa.VARIANT_STREAM),
+                                          ((7, 13, 3, 2), fa.VARIANT_ITEMS),
+                                          ((8, 7, 2, 3), fa.VARIANT_ITEMS),
+                                          ((4, 23, 2, 1), fa.VARIANT_ITEMS),
+                                          ((13, 2, 2, 2), fa.VARIANT_DIRECT)])
+def test_auto_kernel_choice_bitexact(torch_cuda, shape, kernel):
+    """AUTO takes the streaming kernel when mx splits into 3- or 4-element
+    items and the work-list kernel (ragged rows) otherwise."""
+    n, mx, my, mz = shape
+    print("case auto kernel", shape, kernel, flush=True)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    assert f.get_option(fa.OPT_ACTIVE_KERNEL) == kernel
+    f.close()
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=404)
+    assert_same(run_gpu(torch_cuda, u, shape, nshare=2), run_oracle(u, shape, nshare=2), shape)
+
+
+@pytest.mark.parametrize("ps_pad", [0, 512])
+def test_items_padded_plane_stride(torch_cuda, ps_pad):
+    shape = (6, 7, 3, 3)
+    n, mx, my, mz = shape
+    ps = n ** 3 * mx * my + ps_pad if ps_pad else 0
+    print("case items padded", ps, flush=True)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=405, ps=ps)
+    got = run_gpu(torch_cuda, u, shape, ps=ps, variant=(fa.VARIANT_ITEMS, 2))
+    assert_same(got, run_oracle(u, shape, ps=ps), shape, ps=ps)
