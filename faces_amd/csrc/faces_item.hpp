@@ -159,12 +159,12 @@ __device__ __forceinline__ void item_sums(double *lds, int ecnt)
   }
 }
 
-// A run of `count` items of one grid, it0, it0 + stride, ...: item t is
+// A run of `count` items of one grid, item_of(0), item_of(1), ...: item t is
 // summed while the loads of item t+1 are in flight.  Operations are issued
 // per item as J loads (one item ahead), then J stores; J dropped stores
 // before the loop enter it in that same state, so the waits are exact.
-template <int N, int S>
-__device__ __forceinline__ void stream_run(const BatchGrid &G, int it0, int stride, int count, double *lds)
+template <int N, int S, typename Order>
+__device__ __forceinline__ void stream_run(const BatchGrid &G, const Order &item_of, int count, double *lds)
 {
   using SH = StreamShape<N, S>;
   constexpr int J = SH::J;
@@ -176,7 +176,7 @@ __device__ __forceinline__ void stream_run(const BatchGrid &G, int it0, int stri
   double v[J];
   {
     unsigned ld[J], st[J];
-    item_offsets<N, S>(G, it0, ld, st);
+    item_offsets<N, S>(G, item_of(0), ld, st);
 #pragma unroll
     for (int j = 0; j < J; j++)
       v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, ld[j], 0, 0));
@@ -184,7 +184,7 @@ __device__ __forceinline__ void stream_run(const BatchGrid &G, int it0, int stri
 #pragma unroll
   for (int j = 0; j < J; j++)
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, 0.0), rsrc, kOOB - 8 * j, 0, 0);
-  int it = it0;
+  int it = item_of(0);
   for (int t = 0; t < count; t++) {
     __syncthreads();  // the previous item's stores (or the previous run) have read LDS
 #pragma unroll
@@ -192,7 +192,7 @@ __device__ __forceinline__ void stream_run(const BatchGrid &G, int it0, int stri
     // the next item's loads (the last item re-reads its own locations: in
     // bounds, never used); offsets are recomputed where they are used, which
     // keeps only the loaded values live across the item
-    const int nit = (t + 1 < count) ? it + stride : it;
+    const int nit = (t + 1 < count) ? item_of(t + 1) : it;
     {
       unsigned ld[J], st[J];
       item_offsets<N, S>(G, nit, ld, st);
