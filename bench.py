@@ -68,6 +68,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
     ap.add_argument("--grids", type=int, default=256, help="grids in the batched leg")
     ap.add_argument("--requests", type=int, default=2000, help="requests in the served leg")
+    ap.add_argument("--streams", type=int, default=2, help="launch slots of the served leg")
     return ap.parse_args()
 
 
@@ -140,7 +141,7 @@ def batched_leg(torch, fa, ngrids, steps, warmup):
             "tuned": {"seg_len": tuned["seg_len"], "blocks_per_cu": tuned["blocks_per_cu"]}}
 
 
-def served_leg(torch, fa, nreq):
+def served_leg(torch, fa, nreq, args_nstreams=2):
     import random
     from faces_amd.batch import Server
     rng = random.Random(20)
@@ -157,7 +158,7 @@ def served_leg(torch, fa, nreq):
         fields.append(d)
     torch.cuda.synchronize()
     cap = 200e9
-    srv = Server(footprint_cap=cap, max_batch=256, nstreams=2)
+    srv = Server(footprint_cap=cap, max_batch=256, nstreams=args_nstreams)
     # warm the server's launch path on a few requests first
     warm = srv.run_trace([0.0, 10.0, 20.0], shapes[:3], fields[:3])
     del warm
@@ -174,6 +175,7 @@ def served_leg(torch, fa, nreq):
             "req_per_s": round(st["req_per_s"], 1), "p50_us": round(st["p50_us"], 1),
             "p99_us": round(st["p99_us"], 1), "max_us": round(st["max_us"], 1),
             "launches": int(st["launches"]), "footprint_cap_bytes": cap,
+            "streams": args_nstreams,
             "GBs": round(nbytes / st["wall_s"] / 1e9, 2)}
 
 
@@ -280,7 +282,7 @@ def main():
         cpu = cpu_baseline(n, mx, my, mz, kind, args.cpu_seconds)
 
     batched = None if args.no_batch else batched_leg(torch, fa, args.grids, 50, 5)
-    served = None if args.no_serve else served_leg(torch, fa, args.requests)
+    served = None if args.no_serve else served_leg(torch, fa, args.requests, args.streams)
 
     per_sorted = sorted(per)
     rec = {

@@ -37,6 +37,9 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Requests at or above this footprint get a launch of their own.
+constexpr double kLargeBytes = 64.0 * 1024 * 1024;
+
 double now_us(Clock::time_point t0) { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
 
 struct Request {
@@ -90,8 +93,12 @@ int faces_server::launch(Slot &s, std::vector<Request *> &batch)
 {
   std::vector<BatchGrid> g;
   std::vector<Request *> other;
+  // A launch of one large request takes the plain step's kernel choice (the
+  // streaming kernel where the rows split evenly), measured faster on large
+  // grids than the work-list kernel.
+  const bool alone = batch.size() == 1 && batch[0]->bytes >= kLargeBytes;
   for (Request *r : batch) {
-    if (r->boundary == FACES_BOUNDARY_PERIODIC && r->compute && batch_seg(r->n, r->mx, 64)) {
+    if (!alone && r->boundary == FACES_BOUNDARY_PERIODIC && r->compute && batch_seg(r->n, r->mx, 64)) {
       BatchGrid x;
       x.u = r->u;
       x.n = r->n;
@@ -184,15 +191,20 @@ void faces_server::run()
     // admissions into free slots, in arrival order
     for (Slot &s : slots) {
       if (s.busy || queue.empty()) continue;
+      // A launch holds either one large request or small ones only, so small
+      // requests never wait for a large one that shares their launch.
       std::vector<Request *> batch;
       while (!queue.empty() && (int)batch.size() < max_batch &&
              admitted_bytes + queue.front()->bytes <= cap) {
         Request *r = queue.front();
+        const bool large = r->bytes >= kLargeBytes;
+        if (large && !batch.empty()) break;
         queue.pop_front();
         admitted_bytes += r->bytes;
         peak_bytes = std::max(peak_bytes, admitted_bytes);
         n_admitted++;
         batch.push_back(r);
+        if (large) break;
       }
       if (batch.empty()) break;  // the head waits for footprint to free up
       const int st = launch(s, batch);

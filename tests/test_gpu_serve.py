@@ -138,3 +138,22 @@ def test_wait_unknown_ticket(torch_cuda):
     with pytest.raises(fa.FacesError):
         srv.wait(12345, timeout_s=0.1)
     srv.close()
+
+
+def test_large_requests_launch_alone(torch_cuda):
+    """A request of 64 MiB or more gets a launch of its own; the small ones
+    around it are stepped in other launches, all bit-exact."""
+    print("case serve large", flush=True)
+    big = (7, 30, 30, 30)  # 74 MB
+    shapes = [(4, 3, 3, 3), big, (6, 2, 3, 4), (5, 4, 4, 4), big, (8, 2, 2, 2)]
+    fields = [field(torch_cuda, s, 500 + i) for i, s in enumerate(shapes)]
+    torch_cuda.cuda.synchronize()
+    srv = Server(footprint_cap=8e9, max_batch=64, nstreams=1)
+    ts = [srv.submit(d, *s) for d, s in zip(fields, shapes)]
+    for t in ts:
+        srv.wait(t)
+    assert srv.stats()["launches"] >= 3
+    for i, s in enumerate(shapes):
+        r = oracle.compare(fields[i].cpu().numpy(), oracle_one(s, 500 + i), *s)
+        assert r["ndiff"] == 0, (s, r)
+    srv.close()
