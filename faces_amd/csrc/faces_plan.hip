@@ -227,14 +227,29 @@ int stream_slots(int n, int S)
 
 constexpr int kStreamMaxSlots = 12;
 
-// Elements per item: the largest S in {4,3,2,1} not above the requested
-// length that divides mx and keeps the per-thread tables in registers.
+// Segment lengths the streaming kernel is compiled for, per n: 1-4 for all,
+// and longer ones where n is small (a 4-element item of n=4 is 256 points).
+bool stream_case(int n, int S)
+{
+  switch (S) {
+    case 1: case 2: case 3: case 4: return true;
+    case 6: case 8: return n <= 6;
+    case 16: return n <= 4;
+    case 32: return n == 2;
+    default: return false;
+  }
+}
+
+// Elements per item: the largest compiled S not above the requested length
+// (default per n, measured on large grids) that divides mx and keeps the
+// per-thread tables in registers.
 int stream_seg(const faces_plan *p)
 {
   if (p->n < 2 || p->n > 12) return 0;
-  const int want = p->seg_len > 0 ? p->seg_len : 4;
-  for (int s = want < 4 ? want : 4; s >= 1; s--) {
-    if (p->mx % s) continue;
+  static const int kDefault[13] = {0, 0, 32, 16, 16, 8, 4, 4, 4, 3, 2, 1, 1};
+  const int want = p->seg_len > 0 ? p->seg_len : kDefault[p->n];
+  for (int s = std::min(want, 32); s >= 1; s--) {
+    if (!stream_case(p->n, s) || p->mx % s) continue;
     if (stream_slots(p->n, s) <= kStreamMaxSlots) return s;
   }
   return 0;
@@ -287,8 +302,20 @@ int launch_stream_n(const faces_plan *p, const Geom &G, int S, hipStream_t st)
     case 1: return launch_stream_ns<N, 1>(p, G, st);
     case 2: return launch_stream_ns<N, 2>(p, G, st);
     case 3: return launch_stream_ns<N, 3>(p, G, st);
-    default: return launch_stream_ns<N, 4>(p, G, st);
+    case 4: return launch_stream_ns<N, 4>(p, G, st);
+    default: break;
   }
+  if constexpr (N <= 6) {
+    if (S == 6) return launch_stream_ns<N, 6>(p, G, st);
+    if (S == 8) return launch_stream_ns<N, 8>(p, G, st);
+  }
+  if constexpr (N <= 4) {
+    if (S == 16) return launch_stream_ns<N, 16>(p, G, st);
+  }
+  if constexpr (N == 2) {
+    if (S == 32) return launch_stream_ns<N, 32>(p, G, st);
+  }
+  return fail(FACES_ERR_ARG, "streaming kernel has no n=%d S=%d specialisation", N, S);
 }
 
 int launch_stream(const faces_plan *p, double *u, hipStream_t st)
@@ -329,7 +356,10 @@ int active_kernel(const faces_plan *p)
     case FACES_VARIANT_STREAM: return stream_ok(p) ? FACES_VARIANT_STREAM : FACES_VARIANT_DIRECT;
     case FACES_VARIANT_ITEMS: return items_ok(p) ? FACES_VARIANT_ITEMS : FACES_VARIANT_DIRECT;
     default:
-      if (stream_ok(p) && stream_seg(p) >= 3) return FACES_VARIANT_STREAM;
+      // the work-list kernel only where its items (ragged rows allowed) are
+      // more than twice as long as the streaming kernel's
+      if (stream_ok(p) && !(items_ok(p) && batch_seg(p->n, p->mx, 64) > 2 * stream_seg(p)))
+        return FACES_VARIANT_STREAM;
       if (items_ok(p)) return FACES_VARIANT_ITEMS;
       return stream_ok(p) ? FACES_VARIANT_STREAM : FACES_VARIANT_DIRECT;
   }

@@ -98,7 +98,8 @@ int faces_plan_set_tuning(faces_plan *plan, int block, int seg_len, int xcd_rema
  * kernel with item-invariant offset tables), FACES_VARIANT_ITEMS (the
  * batched step's work-list kernel on this one grid: element segments of up
  * to 32 elements, the row's last one shorter, so any mx works); AUTO picks
- * STREAM when mx has a divisor of 3 or 4 elements per item, else ITEMS.
+ * ITEMS when its items are more than twice as long as STREAM's (mx without
+ * a suitable divisor), else STREAM.
  * block: threads per block of the LDS kernel (64..512, multiple of 64),
  * persistent blocks per CU of the STREAM kernel (1..32, or -1 for one work
  * item per block) or of the ITEMS kernel (1..4); 0 keeps the current value. */

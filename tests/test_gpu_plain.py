@@ -313,8 +313,8 @@ def test_invalid_arguments(torch_cuda):
                                           ((4, 23, 2, 1), fa.VARIANT_ITEMS),
                                           ((13, 2, 2, 2), fa.VARIANT_DIRECT)])
 def test_auto_kernel_choice_bitexact(torch_cuda, shape, kernel):
-    """AUTO takes the streaming kernel when mx splits into 3- or 4-element
-    items and the work-list kernel (ragged rows) otherwise."""
+    """AUTO takes the work-list kernel (ragged rows) when its items are more
+    than twice as long as the streaming kernel's, else the streaming kernel."""
     n, mx, my, mz = shape
     print("case auto kernel", shape, kernel, flush=True)
     f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
@@ -333,3 +333,25 @@ def test_items_padded_plane_stride(torch_cuda, ps_pad):
     u = oracle.new_field(*shape, fa.FILL_RAND, seed=405, ps=ps)
     got = run_gpu(torch_cuda, u, shape, ps=ps, variant=(fa.VARIANT_ITEMS, 2))
     assert_same(got, run_oracle(u, shape, ps=ps), shape, ps=ps)
+
+
+@pytest.mark.parametrize("shape,seg", [((4, 32, 2, 2), 16), ((2, 64, 2, 1), 32), ((3, 16, 3, 2), 16),
+                                       ((6, 16, 2, 2), 8), ((5, 12, 3, 1), 6), ((6, 24, 1, 2), 6),
+                                       ((5, 8, 2, 2), 8), ((4, 16, 1, 1), 8)])
+@pytest.mark.parametrize("walk", [0, 1, 2, 3])
+def test_stream_long_segments_bitexact(torch_cuda, shape, seg, walk):
+    """Long items of the streaming kernel for small n."""
+    n, mx, my, mz = shape
+    print("case stream long segments", shape, seg, walk, flush=True)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=406)
+    d = to_dev(torch_cuda, u)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+    f.set_option(fa.OPT_SEG, seg)
+    f.set_option(fa.OPT_STREAM_WALK, walk)
+    f.set_option(fa.OPT_STREAM_ROWS, 2)
+    for _ in range(2):
+        f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    assert_same(got, run_oracle(u, shape, nshare=2), shape)
