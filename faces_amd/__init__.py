@@ -21,7 +21,7 @@ import re
 __all__ = [
     "FacesError", "Faces", "Batch", "Server", "fill", "field_doubles", "lib",
     "LIB_PATH", "OPEN", "PERIODIC", "PATH_AUTO", "PATH_FUSED", "PATH_STAGED",
-    "PATH_STAGED_DIRECT", "FILL_INT", "FILL_RAND", "FILL_REF", "header_symbols",
+    "PATH_STAGED_DIRECT", "PATH_SIX", "FILL_INT", "FILL_RAND", "FILL_REF", "header_symbols",
 ]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -30,7 +30,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libfaces_amd.so")
 HEADER = os.path.join(ROOT, "include", "faces_amd.h")
 
 OPEN, PERIODIC = 0, 1
-PATH_AUTO, PATH_FUSED, PATH_STAGED, PATH_STAGED_DIRECT = 0, 1, 2, 3
+PATH_AUTO, PATH_FUSED, PATH_STAGED, PATH_STAGED_DIRECT, PATH_SIX = 0, 1, 2, 3, 4
 VARIANT_AUTO, VARIANT_DIRECT, VARIANT_LDS, VARIANT_STREAM, VARIANT_ITEMS = 0, 1, 2, 3, 4
 (OPT_VARIANT, OPT_DIRECT_BLOCK, OPT_SEG, OPT_XCD, OPT_LDS_BLOCK, OPT_STREAM_BPC,
  OPT_STREAM_WALK, OPT_ACTIVE_KERNEL, OPT_STREAM_ROWS) = range(9)

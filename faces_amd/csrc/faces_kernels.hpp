@@ -704,6 +704,50 @@ __host__ __device__ inline uint64_t hash64(uint64_t seed, uint64_t idx)
   return z ^ (z >> 31);
 }
 
+// ---------------------------------------------------------------------------
+// The 6-message schedule (gpu/v-6-messages-basic/Faces.cc:65-172): one phase
+// per axis, z then y then x.  A lane owns one face pair across an element
+// boundary along the axis -- the low face of element j and the high face of
+// element j-1 (j = 0: the rank boundary pair, element m-1's high face, whose
+// copies are the self-halo received faces) -- and writes both copies, so the
+// phase runs in place without races.  Faces carry the sums of the earlier
+// phases forward (edges and corners).  lo/hi: the low/high neighbour exists;
+// compute = 0 sums internal pairs only on the rank surface.
+// ---------------------------------------------------------------------------
+static __global__ __launch_bounds__(256) void k_six(Geom G, int axis, int lo, int hi, int compute)
+{
+  const int n = G.n, nm1 = n - 1;
+  const int m[3] = {G.mx, G.my, G.mz};
+  const int b1 = (axis == 0) ? 1 : 0, b2 = (axis == 2) ? 1 : 2;
+  const long total = (long)n * n * m[b1] * m[b2] * m[axis];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  long r = t;
+  const int i1 = (int)(r % n); r /= n;
+  const int i2 = (int)(r % n); r /= n;
+  const int q1 = (int)(r % m[b1]); r /= m[b1];
+  const int q2 = (int)(r % m[b2]); r /= m[b2];
+  const int j = (int)r;
+  const long es[3] = {(long)n * n * n, G.ys, G.ps};  // element strides
+  const long ps_[3] = {1, n, (long)n * n};            // point strides
+  const long base = q1 * es[b1] + q2 * es[b2] + i1 * ps_[b1] + i2 * ps_[b2];
+  const long ia = base + (long)j * es[axis];
+  const long ib = base + (long)(j == 0 ? m[axis] - 1 : j - 1) * es[axis] + (long)nm1 * ps_[axis];
+  const double a = G.u[ia], b = G.u[ib];
+  if (j > 0) {
+    const bool surf = (q1 == 0 && i1 == 0) || (q1 == m[b1] - 1 && i1 == nm1) || (q2 == 0 && i2 == 0) ||
+                      (q2 == m[b2] - 1 && i2 == nm1);
+    if (compute || surf) {
+      const double sum = a + b;
+      G.u[ia] = sum;
+      G.u[ib] = sum;
+    }
+  } else {
+    if (lo) G.u[ia] = a + b;
+    if (hi) G.u[ib] = b + a;
+  }
+}
+
 // kind 0: integer-valued in [0,1023]; 1: uniform [-1,1); 2: the reference's
 // positional init jz*1e4+jy*100+jx+iz*0.01+iy*1e-4+ix*1e-6 (gpu/main.cc:10-19),
 // evaluated left to right without contraction.

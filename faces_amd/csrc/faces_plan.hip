@@ -599,7 +599,7 @@ int faces_plan_destroy(faces_plan *p)
 int faces_plan_set_path(faces_plan *p, int path)
 {
   if (!p) return fail(FACES_ERR_ARG, "null plan");
-  if (path < FACES_PATH_AUTO || path > FACES_PATH_STAGED_DIRECT)
+  if (path < FACES_PATH_AUTO || path > FACES_PATH_SIX)
     return fail(FACES_ERR_ARG, "unknown path %d", path);
   if (path == FACES_PATH_FUSED && p->boundary == FACES_BOUNDARY_OPEN && p->mask != 0)
     return fail(FACES_ERR_ARG, "fused path needs a periodic grid or a rank without neighbours");
@@ -756,6 +756,19 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
   if (path == FACES_PATH_AUTO)
     path = (p->boundary == FACES_BOUNDARY_PERIODIC || p->mask == 0) ? FACES_PATH_FUSED
                                                                     : FACES_PATH_STAGED_DIRECT;
+  if (path == FACES_PATH_SIX) {
+    // z, y, x phases; self halos on the sides where a neighbour exists
+    const int mask = (p->boundary == FACES_BOUNDARY_PERIODIC) ? 63 : p->mask;
+    Geom G = make_geom(p, u);
+    const int m[3] = {p->mx, p->my, p->mz};
+    for (int axis = 2; axis >= 0; axis--) {
+      const long total = (long)p->n * p->n * m[0] * m[1] * m[2];
+      k_six<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(
+          G, axis, (mask >> (2 * axis)) & 1, (mask >> (2 * axis + 1)) & 1, compute ? 1 : 0);
+      FACES_HIP(hipGetLastError());
+    }
+    return FACES_OK;
+  }
   if (path == FACES_PATH_FUSED) {
     if (p->boundary == FACES_BOUNDARY_PERIODIC) {
       // One launch: every node with its wrap partners, in place.

@@ -50,6 +50,11 @@ extern "C" {
 #define FACES_PATH_FUSED 1         /* gather-sum-scatter in place, one launch */
 #define FACES_PATH_STAGED 2        /* pack -> exchange copy -> inner + unpack */
 #define FACES_PATH_STAGED_DIRECT 3 /* pack straight into receive slots */
+#define FACES_PATH_SIX 4           /* the 6-message schedule: z, y, x face phases
+                                      (gpu/v-6-messages-basic/Faces.cc:65-172);
+                                      its summation tree is z-first, so random
+                                      fields differ from the other paths by
+                                      rounding */
 
 /* Field generators (faces_fill). */
 #define FACES_FILL_INT 0  /* integer-valued, uniform in [0,1023] */

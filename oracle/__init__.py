@@ -39,6 +39,8 @@ def lib() -> ctypes.CDLL:
         L.orc_share_staged.restype = i
         L.orc_share_mugs26.argtypes = [dp, i, i, i, i, l, i, i]
         L.orc_share_mugs26.restype = i
+        L.orc_share_six.argtypes = [dp, i, i, i, i, l, i, i]
+        L.orc_share_six.restype = i
         L.orc_pack.argtypes = [dp, i, i, i, i, l, dp, ctypes.POINTER(l)]
         L.orc_pack.restype = i
         L.orc_compare.argtypes = [dp, dp, i, i, i, i, l, dp]
@@ -74,6 +76,13 @@ def fill(u: np.ndarray, n, mx, my, mz, kind=FILL_INT, seed=42, ps=0) -> None:
 def share_sweep(u: np.ndarray, n, mx, my, mz, periodic=True, compute=True, ps=0) -> None:
     """In place: gpu/Mugs.cc:136-203 (+ periodic wrap pairs)."""
     lib().orc_share_sweep(_ptr(u), n, mx, my, mz, ps, int(periodic), int(compute))
+
+
+def share_six(u: np.ndarray, n, mx, my, mz, nbr_mask=63, compute=True, ps=0) -> None:
+    """In place: the 6-message schedule, z then y then x phases
+    (gpu/v-6-messages-basic/Faces.cc:65-172), self halos per nbr_mask."""
+    if lib().orc_share_six(_ptr(u), n, mx, my, mz, ps, nbr_mask, int(compute)) != 0:
+        raise ValueError("orc_share_six: bad arguments")
 
 
 def share_staged(u: np.ndarray, n, mx, my, mz, nbr_mask=63, compute=True, ps=0) -> None:

@@ -187,6 +187,64 @@ void orc_share_sweep(double *u, int n, int mx, int my, int mz, long ps, int peri
 }
 
 /* ------------------------------------------------------------------------ */
+/* the 6-message schedule: gpu/v-6-messages-basic/Faces.cc:65-172             */
+/* ------------------------------------------------------------------------ */
+/*
+ * Three phases, z then y then x.  A phase sums every face pair across an
+ * element boundary along its axis (Faces.cc:112-119 for z): the high face of
+ * element j-1 with the low face of element j, both copies set to the sum.  At
+ * the rank boundary the low face of element 0 adds the face received from
+ * the low neighbour, and the high face of element m-1 the one from the high
+ * neighbour (Faces.cc:114,119); with self halos (nbr_mask bit 2a = low,
+ * 2a+1 = high neighbour exists along axis a) those are this field's own
+ * opposite faces as packed before the phase.  Faces carry the edges and
+ * corners summed by the earlier phases forward, so a node's tree is
+ * ((z pairs) then y pairs) then x pairs.  compute = 0 sums only pairs on the
+ * rank surface (the reference's 6-message variants ignore the flag,
+ * Faces.cc:65; the 26-message predicate, gpu/Mugs.cc:146-150, is kept here).
+ */
+static void six_phase(double *u, const grid_t *g, int axis, int lo, int hi, int compute)
+{
+  const int n = g->n, nm1 = n - 1;
+  const int m[3] = {g->mx, g->my, g->mz};
+  const int ma = m[axis];
+  const int b1 = (axis == 0) ? 1 : 0, b2 = (axis == 2) ? 1 : 2; /* the other two axes */
+#pragma omp parallel for collapse(3) schedule(static)
+  for (int q2 = 0; q2 < m[b2]; q2++)
+    for (int q1 = 0; q1 < m[b1]; q1++)
+      for (int j = 0; j < ma; j++) {
+        for (int i2 = 0; i2 < n; i2++)
+          for (int i1 = 0; i1 < n; i1++) {
+            int jl[3], jh[3], il[3], ih[3];
+            jl[b1] = jh[b1] = q1; jl[b2] = jh[b2] = q2;
+            il[b1] = ih[b1] = i1; il[b2] = ih[b2] = i2;
+            jl[axis] = j; il[axis] = 0;                          /* low face of element j */
+            jh[axis] = (j == 0) ? ma - 1 : j - 1; ih[axis] = nm1; /* high face below it */
+            const long ia = IX(g, il[0], il[1], il[2], jl[0], jl[1], jl[2]);
+            const long ib = IX(g, ih[0], ih[1], ih[2], jh[0], jh[1], jh[2]);
+            if (j > 0) {
+              const int surf = (q1 == 0 && i1 == 0) || (q1 == m[b1] - 1 && i1 == nm1) ||
+                               (q2 == 0 && i2 == 0) || (q2 == m[b2] - 1 && i2 == nm1);
+              if (compute || surf) pair_sum(u, ia, ib);
+            } else {
+              const double a = u[ia], b = u[ib];
+              if (lo) u[ia] = a + b;
+              if (hi) u[ib] = b + a;
+            }
+          }
+      }
+}
+
+int orc_share_six(double *u, int n, int mx, int my, int mz, long ps, int nbr_mask, int compute)
+{
+  if (n < 2 || mx < 1 || my < 1 || mz < 1 || nbr_mask < 0 || nbr_mask > 63) return -1;
+  grid_t g = mkgrid(n, mx, my, mz, ps);
+  for (int axis = 2; axis >= 0; axis--)
+    six_phase(u, &g, axis, (nbr_mask >> (2 * axis)) & 1, (nbr_mask >> (2 * axis + 1)) & 1, compute);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* the 26 messages (gpu/v-26-messages/Faces.cc:37-75, 243-293)               */
 /* ------------------------------------------------------------------------ */
 /*

@@ -355,3 +355,55 @@ def test_stream_long_segments_bitexact(torch_cuda, shape, seg, walk):
     got = to_host(torch_cuda, d)
     f.close()
     assert_same(got, run_oracle(u, shape, nshare=2), shape)
+
+
+# ------------------------------------------- the 6-message schedule (z, y, x)
+
+SIX_SHAPES = [(2, 1, 1, 1), (4, 3, 2, 2), (5, 1, 3, 2), (7, 4, 3, 5), (8, 2, 2, 2), (12, 2, 1, 3),
+              (13, 2, 2, 1)]
+
+
+@pytest.mark.parametrize("shape", SIX_SHAPES)
+@pytest.mark.parametrize("kind", [fa.FILL_INT, fa.FILL_RAND])
+@pytest.mark.parametrize("compute", [True, False])
+def test_six_periodic_bitexact(torch_cuda, shape, kind, compute):
+    """PATH_SIX against the oracle's 6-message schedule (bit for bit, random
+    fields too) and, on integer fields, against the 26-message result."""
+    n, mx, my, mz = shape
+    print("case six periodic", shape, kind, compute, flush=True)
+    u = oracle.new_field(*shape, kind, seed=501)
+    got = run_gpu(torch_cuda, u, shape, compute=compute, path=fa.PATH_SIX, nshare=2)
+    want = u.copy()
+    for _ in range(2):
+        oracle.share_six(want, *shape, nbr_mask=63, compute=compute)
+    assert_same(got, want, shape)
+    if kind == fa.FILL_INT:
+        assert_same(got, run_oracle(u, shape, compute=compute, nshare=2), shape)
+
+
+@pytest.mark.parametrize("lgrid,rank", [((3, 3, 3), 13), ((3, 3, 3), 0), ((3, 3, 3), 4),
+                                        ((2, 1, 3), 3), ((1, 1, 1), 0)])
+@pytest.mark.parametrize("kind", [fa.FILL_INT, fa.FILL_RAND])
+def test_six_open_ranks_bitexact(torch_cuda, lgrid, rank, kind):
+    shape = (4, 3, 2, 2)
+    n, mx, my, mz = shape
+    print("case six open", lgrid, rank, kind, flush=True)
+    u = oracle.new_field(*shape, kind, seed=502)
+    got = run_gpu(torch_cuda, u, shape, boundary=fa.OPEN, path=fa.PATH_SIX, rank=rank, lgrid=lgrid)
+    mask = brute.mask_for_rank(rank, *lgrid)
+    want = u.copy()
+    oracle.share_six(want, *shape, nbr_mask=mask)
+    assert_same(got, want, shape)
+    if kind == fa.FILL_INT:
+        ref = u.copy()
+        oracle.share_staged(ref, *shape, nbr_mask=mask)
+        assert_same(got, ref, shape)
+
+
+def test_six_c2_size_integer(torch_cuda):
+    """60^3 n=7 integer field, 3 shares of PATH_SIX equal the oracle sweep."""
+    shape = (7, 60, 60, 60)
+    print("case six c2", flush=True)
+    u = oracle.new_field(*shape, fa.FILL_INT, seed=42)
+    got = run_gpu(torch_cuda, u, shape, path=fa.PATH_SIX, nshare=3)
+    assert_same(got, run_oracle(u, shape, nshare=3), shape)

@@ -349,3 +349,64 @@ def test_planted_transposed_index_is_detected():
     good = u.copy()
     oracle.share_sweep(good, *shape, periodic=True)
     assert np.array_equal(good, want)
+
+
+# ------------------------------------------- the 6-message schedule (z, y, x)
+
+@pytest.mark.parametrize("shape", [(4, 3, 2, 2), (3, 1, 2, 3), (5, 2, 2, 2), (2, 1, 1, 1),
+                                   (6, 3, 1, 2)])
+@pytest.mark.parametrize("compute", [True, False])
+def test_six_equals_sweep_and_staged_on_integers(shape, compute):
+    """gpu/v-6-messages-basic/Faces.cc:65-172 sums the same copies as the
+    26-message step: on integer fields every order gives the same bits."""
+    u = oracle.new_field(*shape, oracle.FILL_INT, seed=7)
+    a = u.copy()
+    oracle.share_six(a, *shape, nbr_mask=63, compute=compute)
+    b = u.copy()
+    oracle.share_sweep(b, *shape, periodic=True, compute=compute)
+    assert np.array_equal(a, b)
+    for mask in range(64):
+        a = u.copy()
+        oracle.share_six(a, *shape, nbr_mask=mask, compute=compute)
+        b = u.copy()
+        oracle.share_staged(b, *shape, nbr_mask=mask, compute=compute)
+        assert np.array_equal(a, b), mask
+
+
+def test_six_matches_bruteforce_integer():
+    shape = (4, 3, 2, 3)
+    u = oracle.new_field(*shape, oracle.FILL_INT, seed=17)
+    a = u.copy()
+    oracle.share_six(a, *shape)
+    want = brute.share(brute.as6(u, *shape), *shape, periodic=True).ravel()
+    assert np.array_equal(a, want)
+
+
+def test_six_tree_is_z_then_y_then_x():
+    """One periodic element of n=2: all 8 points are copies of one node.  The
+    z phase pairs iz, then y pairs iy, then x pairs ix."""
+    u = oracle.new_field(2, 1, 1, 1, oracle.FILL_RAND, seed=21)
+    c = u.reshape(2, 2, 2)  # [iz][iy][ix]
+    z = [[c[0, iy, ix] + c[1, iy, ix] for ix in range(2)] for iy in range(2)]
+    y = [z[0][ix] + z[1][ix] for ix in range(2)]
+    want = y[0] + y[1]
+    a = u.copy()
+    oracle.share_six(a, 2, 1, 1, 1)
+    assert np.all(a == want)
+    # the 26-message tree (x, then y, then z) differs on these values
+    b = u.copy()
+    oracle.share_sweep(b, 2, 1, 1, 1, periodic=True)
+    x = [[c[iz, iy, 0] + c[iz, iy, 1] for iy in range(2)] for iz in range(2)]
+    assert np.all(b == (x[0][0] + x[0][1]) + (x[1][0] + x[1][1]))
+
+
+def test_six_random_within_rounding():
+    shape = (5, 3, 2, 2)
+    u = oracle.new_field(*shape, oracle.FILL_RAND, seed=23)
+    a = u.copy()
+    oracle.share_six(a, *shape)
+    exact = brute.share(brute.as6(u, *shape), *shape, periodic=True).ravel()
+    mult = brute.multiplicity(*shape, periodic=True).ravel()
+    absum = brute.share(brute.as6(np.abs(u), *shape), *shape, periodic=True).ravel()
+    bound = (mult - 1) * np.finfo(np.float64).eps * absum
+    assert np.all(np.abs(a - exact) <= bound + 1e-300)
