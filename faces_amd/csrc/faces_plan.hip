@@ -66,6 +66,7 @@ struct faces_plan {
   int stream_walk;   // streaming kernel traversal: 0 items, 1 x walk, 2 y walk, 3 row groups
   int stream_rows;   // walk 3: rows per group (the largest divisor of my not above it)
   int items_bpc;     // work-list item kernel: persistent blocks per CU
+  int stream_planes; // 1: per-plane descriptors even below 4 GB
   // message arena: send slots then receive slots
   double *arena;
   long arena_doubles;
@@ -255,11 +256,17 @@ int stream_seg(const faces_plan *p)
   return 0;
 }
 
-// Byte offsets of the streaming kernel are 32-bit.
+// Byte offsets of the streaming kernel are 32-bit: within the field, or,
+// for fields of 4 GB and more, within one jz plane.
+bool stream_big(const faces_plan *p)
+{
+  return p->stream_planes || 8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps) >= 0xFFFFFFF0L;
+}
+
 bool stream_ok(const faces_plan *p)
 {
-  const long bytes = 8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps);
-  return stream_seg(p) > 0 && bytes < 0xFFFFFFF0L;
+  const long plane = 8L * p->n * p->n * p->n * p->mx * p->my;
+  return stream_seg(p) > 0 && plane < 0xFFFFFFF0L;
 }
 
 template <int N, int S>
@@ -270,7 +277,9 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, hipStream_t st)
   I.mz = p->mz;
   I.nseg = p->mx / S;
   I.nitems = p->my * p->mz * I.nseg;
-  I.field_bytes = (unsigned)(8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps));
+  const bool big = stream_big(p);
+  I.field_bytes = big ? (unsigned)(8L * p->n * p->n * p->n * p->mx * p->my)
+                      : (unsigned)(8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps));
   I.xcd_remap = p->xcd_remap;
   I.walk = p->stream_walk;
   I.rows = 1;
@@ -290,7 +299,10 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, hipStream_t st)
   if (I.xcd_remap) nb = (nb + 7) / 8 * 8;
   const size_t lds = stream_lds_doubles<N, S>(I.walk, I.rows) * sizeof(double);
   if (lds > 64 * 1024) return fail(FACES_ERR_ARG, "streaming kernel LDS %zu B above 64 KiB", lds);
-  k_stream<N, S><<<dim3((unsigned)nb), dim3(kStreamThreads), lds, st>>>(G, I);
+  if (big)
+    k_stream<N, S, true><<<dim3((unsigned)nb), dim3(kStreamThreads), lds, st>>>(G, I);
+  else
+    k_stream<N, S, false><<<dim3((unsigned)nb), dim3(kStreamThreads), lds, st>>>(G, I);
   FACES_HIP(hipGetLastError());
   return FACES_OK;
 }
@@ -544,6 +556,7 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
   p->stream_walk = 1;
   p->stream_rows = 4;
   p->items_bpc = 4;
+  p->stream_planes = 0;
 
   // Message sizes, gpu/v-26-messages/Faces.cc:44-47, 64-66.
   const long nn = (long)n * n;
@@ -680,6 +693,9 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       if (value < 1 || value > 64) return fail(FACES_ERR_ARG, "rows per group %d outside [1,64]", value);
       p->stream_rows = value;
       return FACES_OK;
+    case FACES_OPT_STREAM_PLANES:
+      p->stream_planes = value ? 1 : 0;
+      return FACES_OK;
     default: return fail(FACES_ERR_ARG, "unknown option %d", option);
   }
 }
@@ -696,6 +712,7 @@ int faces_plan_get_option(const faces_plan *p, int option, int *value)
     case FACES_OPT_STREAM_BPC: *value = p->stream_bpc; break;
     case FACES_OPT_STREAM_WALK: *value = p->stream_walk; break;
     case FACES_OPT_STREAM_ROWS: *value = p->stream_rows; break;
+    case FACES_OPT_STREAM_PLANES: *value = stream_big(p) ? 1 : 0; break;
     case FACES_OPT_ACTIVE_KERNEL: *value = active_kernel(p); break;
     default: return fail(FACES_ERR_ARG, "unknown option %d", option);
   }

@@ -57,7 +57,7 @@ struct StreamItems {
   int my, mz;
   int nseg;             // mx / S
   int nitems;           // my*mz*nseg
-  unsigned field_bytes; // below 2^32 - 8: offsets are 32-bit
+  unsigned field_bytes; // below 2^32 - 8: offsets are 32-bit (BIG kernels: one plane's bytes)
   int xcd_remap;        // 1: contiguous ranges per XCD
   int walk;             // 0 items, 1 rows in x (seam carried), 2 columns in y (y rows carried),
                         // 3 groups of `rows` rows, segment by segment (seam and y rows carried)
@@ -73,7 +73,12 @@ __host__ __device__ constexpr int stream_lds_doubles(int walk, int rows)
          ((walk == 1) ? 1 : (walk == 3) ? rows : 0) * StreamShape<N, S>::cS;
 }
 
-template <int N, int S>
+// BIG: the field spans 4 GB or more.  Each item then addresses its two
+// planes (jz and jz-1, or the wrapped plane) through a descriptor of its own
+// with 32-bit offsets inside the plane; slots of the partner region that may
+// lie in either plane are loaded and stored once through each descriptor,
+// the lanes of the other plane sent out of range.
+template <int N, int S, bool BIG = false>
 __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I)
 {
   using SH = StreamShape<N, S>;
@@ -263,6 +268,13 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
 
   unsigned obase = 0;          // item's OWN base (bytes, block-uniform)
   unsigned pbase[SH::JP];      // item's per-lane PART bases (bytes)
+  // BIG: descriptors of the loaded item's planes jz (A) and jz-1 (B), and
+  // the PART slots whose region lies in plane B
+  __amdgpu_buffer_rsrc_t rA = rsrc, rB = rsrc;
+  unsigned inB = 0;
+#pragma unroll
+  for (int jp = 0; jp < SH::JP; jp++)
+    if ((rowsel[jp] & 3) >= 2) inB |= 1u << jp;
   auto bases = [&](int it) {
     const int sg = it % I.nseg;
     const int row = it / I.nseg;
@@ -271,10 +283,15 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     const int jzm = (jz == 0) ? G.mz - 1 : jz - 1;
     const int jx0 = sg * S;
     const int jxs = (jx0 == 0) ? G.mx - 1 : jx0 - 1;
-    const unsigned r0 = (unsigned)((jz * G.ps + jy * G.ys) * 8);
-    const unsigned r1 = (unsigned)((jz * G.ps + jym * G.ys) * 8);
-    const unsigned r2 = (unsigned)((jzm * G.ps + jy * G.ys) * 8);
-    const unsigned r3 = (unsigned)((jzm * G.ps + jym * G.ys) * 8);
+    const long pa = BIG ? 0 : jz * G.ps, pb = BIG ? 0 : jzm * G.ps;
+    const unsigned r0 = (unsigned)((pa + jy * G.ys) * 8);
+    const unsigned r1 = (unsigned)((pa + jym * G.ys) * 8);
+    const unsigned r2 = (unsigned)((pb + jy * G.ys) * 8);
+    const unsigned r3 = (unsigned)((pb + jym * G.ys) * 8);
+    if constexpr (BIG) {
+      rA = __builtin_amdgcn_make_buffer_rsrc(G.u + jz * G.ps, 0, (int)I.field_bytes, 0x00020000);
+      rB = __builtin_amdgcn_make_buffer_rsrc(G.u + jzm * G.ps, 0, (int)I.field_bytes, 0x00020000);
+    }
     obase = r0 + (unsigned)(jx0 * n3 * 8);
 #pragma unroll
     for (int jp = 0; jp < SH::JP; jp++) {
@@ -293,7 +310,18 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     for (int j = 0; j < J; j++) {
       unsigned off = ((j < JO) ? obase : pbase[j - JO < 0 ? 0 : j - JO]) + voff[j];
       if (j >= JO && (cm & (1u << (j - JO)))) off = kOOB;
-      v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+      if constexpr (!BIG) {
+        v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+      } else if (j < JO) {
+        v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rA, off, 0, 0));
+      } else {
+        const bool b = inB & (1u << (j - JO < 0 ? 0 : j - JO));
+        const double xa =
+            __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rA, b ? kOOB : off, 0, 0));
+        const double xb =
+            __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rB, b ? off : kOOB, 0, 0));
+        v[j] = b ? xb : xa;
+      }
     }
   };
 
@@ -304,12 +332,15 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
   double vcar[SH::JP];
 #pragma unroll
   for (int jp = 0; jp < SH::JP; jp++) vcar[jp] = 0.0;
-  // J dropped stores (offset out of range): the loop is entered with the same
-  // pattern of outstanding operations as every later iteration -- J loads,
-  // then J stores -- so the waits at the loop head are exact.
+  // Dropped stores (offset out of range), as many as an item issues: the loop
+  // is entered with the same pattern of outstanding operations as every later
+  // iteration -- an item's loads, then its stores -- so the waits at the loop
+  // head are exact.
+  constexpr int NST = J + (BIG ? SH::JP : 0);
 #pragma unroll
-  for (int j = 0; j < J; j++)
+  for (int j = 0; j < NST; j++)
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, 0.0), rsrc, kOOB - 8 * j, 0, 0);
+  __amdgpu_buffer_rsrc_t cA = rA, cB = rB;  // the current item's descriptors (BIG)
   while (true) {
     __syncthreads();  // the previous item's stores have read LDS
     const int rr = sg % R;  // the current item's row within the group
@@ -341,6 +372,8 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     // Unconditional loads keep the wait accounting exact; after the last item
     // they re-read its own locations (in bounds, never used).
     const int nit = more ? item_of(nk, ns) : it;
+    cA = rA;
+    cB = rB;
     bases(nit);
     load_item(ncy, ncs);
     __syncthreads();
@@ -391,9 +424,18 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < J; j++)
-      __builtin_amdgcn_raw_buffer_store_b64(
-          __builtin_bit_cast(u32x2, lds[j * kStreamThreads + tid]), rsrc, soff[j], 0, 0);
+    for (int j = 0; j < J; j++) {
+      const u32x2 x = __builtin_bit_cast(u32x2, lds[j * kStreamThreads + tid]);
+      if constexpr (!BIG) {
+        __builtin_amdgcn_raw_buffer_store_b64(x, rsrc, soff[j], 0, 0);
+      } else if (j < JO) {
+        __builtin_amdgcn_raw_buffer_store_b64(x, cA, soff[j], 0, 0);
+      } else {
+        const bool b = inB & (1u << (j - JO < 0 ? 0 : j - JO));
+        __builtin_amdgcn_raw_buffer_store_b64(x, cA, b ? kOOB : soff[j], 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(x, cB, b ? soff[j] : kOOB, 0, 0);
+      }
+    }
     if (!more) break;
     k = nk;
     sg = ns;

@@ -131,7 +131,11 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *                            a group of rows segment by segment, carrying both
  *   FACES_OPT_ACTIVE_KERNEL  (get only) the variant faces_share launches
  *   FACES_OPT_STREAM_ROWS    STREAM walk 3: rows per group (the largest divisor
- *                            of my not above the value is used) */
+ *                            of my not above the value is used)
+ *   FACES_OPT_STREAM_PLANES  STREAM kernel addressing: 1 = through one
+ *                            descriptor per jz plane (what fields of 4 GB and
+ *                            more use; set, it applies to any field); get:
+ *                            whether the plan addresses per plane */
 #define FACES_OPT_VARIANT 0
 #define FACES_OPT_DIRECT_BLOCK 1
 #define FACES_OPT_SEG 2
@@ -141,6 +145,7 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
 #define FACES_OPT_STREAM_WALK 6
 #define FACES_OPT_ACTIVE_KERNEL 7
 #define FACES_OPT_STREAM_ROWS 8
+#define FACES_OPT_STREAM_PLANES 9
 int faces_plan_set_option(faces_plan *plan, int option, int value);
 int faces_plan_get_option(const faces_plan *plan, int option, int *value);
 

@@ -407,3 +407,51 @@ def test_six_c2_size_integer(torch_cuda):
     u = oracle.new_field(*shape, fa.FILL_INT, seed=42)
     got = run_gpu(torch_cuda, u, shape, path=fa.PATH_SIX, nshare=3)
     assert_same(got, run_oracle(u, shape, nshare=3), shape)
+
+
+# --------------------------------- per-plane addressing (fields of 4 GB and more)
+
+@pytest.mark.parametrize("walk,rows", [(0, 1), (1, 1), (2, 1), (3, 2)])
+@pytest.mark.parametrize("shape,seg", [((7, 12, 3, 4), 4), ((4, 32, 2, 3), 16), ((12, 3, 2, 2), 1),
+                                       ((5, 6, 1, 1), 3), ((8, 4, 3, 1), 2)])
+def test_stream_per_plane_bitexact(torch_cuda, walk, rows, shape, seg):
+    """The per-plane descriptors that fields above 4 GB need, forced on small
+    fields (every walk, wrap planes included)."""
+    n, mx, my, mz = shape
+    print("case stream per plane", shape, seg, walk, rows, flush=True)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=601)
+    d = to_dev(torch_cuda, u)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+    f.set_option(fa.OPT_STREAM_PLANES, 1)
+    assert f.get_option(fa.OPT_STREAM_PLANES) == 1
+    f.set_option(fa.OPT_SEG, seg)
+    f.set_option(fa.OPT_STREAM_WALK, walk)
+    f.set_option(fa.OPT_STREAM_ROWS, rows)
+    for _ in range(2):
+        f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    assert_same(got, run_oracle(u, shape, nshare=2), shape)
+
+
+@pytest.mark.slow
+def test_field_above_4gb_bitexact(torch_cuda):
+    """A 4.6 GB field (n=8, 100x100x112 elements) takes the streaming kernel
+    with per-plane descriptors; one share equals the oracle."""
+    shape = (8, 100, 100, 112)
+    n, mx, my, mz = shape
+    print("case field above 4 GB", shape, flush=True)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    assert f.get_option(fa.OPT_ACTIVE_KERNEL) == fa.VARIANT_STREAM
+    assert f.get_option(fa.OPT_STREAM_PLANES) == 1
+    u = oracle.new_field(*shape, fa.FILL_INT, seed=602)
+    assert 8 * u.size > 4 * 2 ** 30
+    d = to_dev(torch_cuda, u)
+    f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    del d
+    torch_cuda.cuda.empty_cache()
+    oracle.share_sweep(u, *shape)
+    assert np.array_equal(got, u)
