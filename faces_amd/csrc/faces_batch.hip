@@ -264,14 +264,19 @@ struct faces_batch {
 
 namespace {
 
-bool batchable(int n) { return n >= 2 && n <= 12; }
+// Grids the work-list kernel takes: n within its routines and a field its
+// 32-bit offsets reach.  The others step on their own plan.
+bool batchable(int n, int mx, int my, int mz)
+{
+  return n >= 2 && n <= 12 && 8 * faces_field_doubles(n, mx, my, mz, 0) < 0xFFFFFFF0L;
+}
 
 int upload(faces_batch *b)
 {
   std::vector<BatchGrid> g;
   b->solo.clear();
   for (int i = 0; i < b->ngrids; i++) {
-    if (!batchable(b->n[i])) {
+    if (!batchable(b->n[i], b->mx[i], b->my[i], b->mz[i])) {
       b->solo.push_back(i);
       continue;
     }
@@ -370,12 +375,6 @@ int faces_batch_create(faces_batch **out, int ngrids, const int *n, const int *m
     if (n[i] < 2 || n[i] > 64 || mx[i] < 1 || my[i] < 1 || mz[i] < 1) {
       delete b;
       return fail(FACES_ERR_ARG, "grid %d: shape n=%d m=%dx%dx%d invalid", i, n[i], mx[i], my[i], mz[i]);
-    }
-    const long bytes = 8 * faces_field_doubles(n[i], mx[i], my[i], mz[i], 0);
-    if (batchable(n[i]) && bytes >= 0xFFFFFFF0L) {
-      delete b;
-      return fail(FACES_ERR_ARG, "grid %d: field of %ld bytes exceeds the batched step's 4 GB limit", i,
-                  bytes);
     }
     total_units += (long)mx[i] * my[i] * mz[i];
     b->n.push_back(n[i]);
@@ -481,7 +480,7 @@ int faces_batch_info(const faces_batch *b, double *bytes, long *items)
   long it = 0;
   for (int i = 0; i < b->ngrids; i++) {
     s += 8.0 * faces_field_doubles(b->n[i], b->mx[i], b->my[i], b->mz[i], 0);
-    const int S = batchable(b->n[i]) ? batch_seg(b->n[i], b->mx[i], b->seg_cap) : 0;
+    const int S = batchable(b->n[i], b->mx[i], b->my[i], b->mz[i]) ? batch_seg(b->n[i], b->mx[i], b->seg_cap) : 0;
     it += S ? (long)b->my[i] * b->mz[i] * ((b->mx[i] + S - 1) / S) : 0;
   }
   *bytes = s;

@@ -771,8 +771,9 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
   hipStream_t s = (hipStream_t)stream;
   int path = p->path;
   if (path == FACES_PATH_AUTO)
-    path = (p->boundary == FACES_BOUNDARY_PERIODIC || p->mask == 0) ? FACES_PATH_FUSED
-                                                                    : FACES_PATH_STAGED_DIRECT;
+    path = (p->boundary == FACES_BOUNDARY_PERIODIC || p->mask == 0 || p->mask == 63)
+               ? FACES_PATH_FUSED
+               : FACES_PATH_STAGED_DIRECT;
   if (path == FACES_PATH_SIX) {
     // z, y, x phases; self halos on the sides where a neighbour exists
     const int mask = (p->boundary == FACES_BOUNDARY_PERIODIC) ? 63 : p->mask;
@@ -787,7 +788,9 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
     return FACES_OK;
   }
   if (path == FACES_PATH_FUSED) {
-    if (p->boundary == FACES_BOUNDARY_PERIODIC) {
+    // An open rank whose six face neighbours all exist, stood in for by self
+    // halos, sums exactly the periodic wrap partners (same tree, same bits).
+    if (p->boundary == FACES_BOUNDARY_PERIODIC || p->mask == 63) {
       // One launch: every node with its wrap partners, in place.
       if (compute) return launch_box(p, u, false, s);
       return launch_surface(p, u, bound_wrap(), s);

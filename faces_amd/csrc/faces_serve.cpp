@@ -98,7 +98,8 @@ int faces_server::launch(Slot &s, std::vector<Request *> &batch)
   // grids than the work-list kernel.
   const bool alone = batch.size() == 1 && batch[0]->bytes >= kLargeBytes;
   for (Request *r : batch) {
-    if (!alone && r->boundary == FACES_BOUNDARY_PERIODIC && r->compute && batch_seg(r->n, r->mx, 64)) {
+    if (!alone && r->boundary == FACES_BOUNDARY_PERIODIC && r->compute && batch_seg(r->n, r->mx, 64) &&
+        r->bytes < 4294967280.0) {
       BatchGrid x;
       x.u = r->u;
       x.n = r->n;
@@ -295,8 +296,6 @@ static int submit_at(faces_server *s, int n, int mx, int my, int mz, int boundar
     return fail(FACES_ERR_ARG, "unknown boundary %d", boundary);
   const double bytes = 8.0 * faces_field_doubles(n, mx, my, mz, 0);
   if (bytes > s->cap) return fail(FACES_ERR_ADMISSION, "request of %.0f bytes exceeds the cap %.0f", bytes, s->cap);
-  if (bytes >= 4294967280.0 && boundary == FACES_BOUNDARY_PERIODIC && compute)
-    return fail(FACES_ERR_ARG, "served fields are limited to 4 GB");
   Request *r = new (std::nothrow) Request();
   if (!r) return fail(FACES_ERR_NOMEM, "request allocation");
   r->n = n;

@@ -228,7 +228,7 @@ def test_rank_positions_open(torch_cuda, lgrid):
         u = oracle.new_field(*shape, fa.FILL_RAND, seed=rank + 1)
         want = u.copy()
         oracle.share_staged(want, *shape, nbr_mask=mask)
-        for path in (fa.PATH_STAGED, fa.PATH_STAGED_DIRECT):
+        for path in (fa.PATH_AUTO, fa.PATH_STAGED, fa.PATH_STAGED_DIRECT):
             got = run_gpu(torch_cuda, u, shape, boundary=fa.OPEN, rank=rank, lgrid=lgrid,
                           path=path)
             assert_same(got, want, shape)
@@ -269,6 +269,13 @@ def test_c3_configuration(torch_cuda):
     assert_same(got, want, shape)
     got2 = run_gpu(torch_cuda, u, shape, rank=0, lgrid=(6, 6, 6))
     assert_same(got2, want, shape)
+    # an interior rank of the open 6x6x6 decomposition with self halos: its
+    # six face neighbours exist, so it sums the periodic partners (fused path)
+    got3 = run_gpu(torch_cuda, u, shape, boundary=fa.OPEN, rank=43, lgrid=(6, 6, 6))
+    assert_same(got3, want, shape)
+    ref = u.copy()
+    oracle.share_staged(ref, *shape, nbr_mask=63)
+    assert_same(got3, ref, shape)
 
 
 def test_c2_configuration(torch_cuda):
