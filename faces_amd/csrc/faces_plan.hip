@@ -270,7 +270,7 @@ bool stream_ok(const faces_plan *p)
 }
 
 template <int N, int S>
-int launch_stream_ns(const faces_plan *p, const Geom &G, hipStream_t st)
+int launch_stream_ns(const faces_plan *p, const Geom &G, bool skip, hipStream_t st)
 {
   StreamItems I;
   I.my = p->my;
@@ -282,6 +282,7 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, hipStream_t st)
                       : (unsigned)(8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps));
   I.xcd_remap = p->xcd_remap;
   I.walk = p->stream_walk;
+  I.skip_surface = skip ? 1 : 0;
   I.rows = 1;
   if (I.walk == 3)
     for (int r = std::max(1, std::min(p->stream_rows, p->my)); r >= 1; r--)
@@ -308,44 +309,44 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, hipStream_t st)
 }
 
 template <int N>
-int launch_stream_n(const faces_plan *p, const Geom &G, int S, hipStream_t st)
+int launch_stream_n(const faces_plan *p, const Geom &G, int S, bool skip, hipStream_t st)
 {
   switch (S) {
-    case 1: return launch_stream_ns<N, 1>(p, G, st);
-    case 2: return launch_stream_ns<N, 2>(p, G, st);
-    case 3: return launch_stream_ns<N, 3>(p, G, st);
-    case 4: return launch_stream_ns<N, 4>(p, G, st);
+    case 1: return launch_stream_ns<N, 1>(p, G, skip, st);
+    case 2: return launch_stream_ns<N, 2>(p, G, skip, st);
+    case 3: return launch_stream_ns<N, 3>(p, G, skip, st);
+    case 4: return launch_stream_ns<N, 4>(p, G, skip, st);
     default: break;
   }
   if constexpr (N <= 6) {
-    if (S == 6) return launch_stream_ns<N, 6>(p, G, st);
-    if (S == 8) return launch_stream_ns<N, 8>(p, G, st);
+    if (S == 6) return launch_stream_ns<N, 6>(p, G, skip, st);
+    if (S == 8) return launch_stream_ns<N, 8>(p, G, skip, st);
   }
   if constexpr (N <= 4) {
-    if (S == 16) return launch_stream_ns<N, 16>(p, G, st);
+    if (S == 16) return launch_stream_ns<N, 16>(p, G, skip, st);
   }
   if constexpr (N == 2) {
-    if (S == 32) return launch_stream_ns<N, 32>(p, G, st);
+    if (S == 32) return launch_stream_ns<N, 32>(p, G, skip, st);
   }
   return fail(FACES_ERR_ARG, "streaming kernel has no n=%d S=%d specialisation", N, S);
 }
 
-int launch_stream(const faces_plan *p, double *u, hipStream_t st)
+int launch_stream(const faces_plan *p, double *u, bool skip, hipStream_t st)
 {
   const Geom G = make_geom(p, u);
   const int S = stream_seg(p);
   switch (p->n) {
-    case 2: return launch_stream_n<2>(p, G, S, st);
-    case 3: return launch_stream_n<3>(p, G, S, st);
-    case 4: return launch_stream_n<4>(p, G, S, st);
-    case 5: return launch_stream_n<5>(p, G, S, st);
-    case 6: return launch_stream_n<6>(p, G, S, st);
-    case 7: return launch_stream_n<7>(p, G, S, st);
-    case 8: return launch_stream_n<8>(p, G, S, st);
-    case 9: return launch_stream_n<9>(p, G, S, st);
-    case 10: return launch_stream_n<10>(p, G, S, st);
-    case 11: return launch_stream_n<11>(p, G, S, st);
-    case 12: return launch_stream_n<12>(p, G, S, st);
+    case 2: return launch_stream_n<2>(p, G, S, skip, st);
+    case 3: return launch_stream_n<3>(p, G, S, skip, st);
+    case 4: return launch_stream_n<4>(p, G, S, skip, st);
+    case 5: return launch_stream_n<5>(p, G, S, skip, st);
+    case 6: return launch_stream_n<6>(p, G, S, skip, st);
+    case 7: return launch_stream_n<7>(p, G, S, skip, st);
+    case 8: return launch_stream_n<8>(p, G, S, skip, st);
+    case 9: return launch_stream_n<9>(p, G, S, skip, st);
+    case 10: return launch_stream_n<10>(p, G, S, skip, st);
+    case 11: return launch_stream_n<11>(p, G, S, skip, st);
+    case 12: return launch_stream_n<12>(p, G, S, skip, st);
     default: return fail(FACES_ERR_ARG, "streaming kernel has no n=%d specialisation", p->n);
   }
 }
@@ -379,8 +380,12 @@ int active_kernel(const faces_plan *p)
 
 int launch_box(const faces_plan *p, double *u, bool skip_surface, hipStream_t st)
 {
-  const int kern = skip_surface ? FACES_VARIANT_DIRECT : active_kernel(p);
-  if (kern == FACES_VARIANT_STREAM) return launch_stream(p, u, st);
+  int kern = active_kernel(p);
+  // interior only (open ranks): the streaming kernel skips the surface
+  // nodes; the other kernels fall back to the direct one
+  if (skip_surface && kern != FACES_VARIANT_STREAM)
+    kern = (p->variant == FACES_VARIANT_AUTO && stream_ok(p)) ? FACES_VARIANT_STREAM : FACES_VARIANT_DIRECT;
+  if (kern == FACES_VARIANT_STREAM) return launch_stream(p, u, skip_surface, st);
   if (kern == FACES_VARIANT_ITEMS)
     return launch_single(u, p->n, p->mx, p->my, p->mz, p->ps, p->seg_len > 0 ? p->seg_len : 64,
                          p->items_bpc, st);

@@ -62,6 +62,8 @@ struct StreamItems {
   int walk;             // 0 items, 1 rows in x (seam carried), 2 columns in y (y rows carried),
                         // 3 groups of `rows` rows, segment by segment (seam and y rows carried)
   int rows;             // walk 3: rows per group (divides my)
+  int skip_surface;     // 1: sum only nodes off the rank surface (open ranks:
+                        // the surface is summed with the received copies)
 };
 
 // LDS doubles of the streaming kernel: the item footprint plus, for the walks
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     }
 #pragma unroll
     for (int w = 0; w < 4; w++) np[i][w] = p[2 * w] | (p[2 * w + 1] << 16);
-    nflag[i] = (ix == 0 ? 1u : 0u) | (iy == 0 ? 2u : 0u) | (iz == 0 ? 4u : 0u) | 8u;
+    nflag[i] = (ix == 0 ? 1u : 0u) | (iy == 0 ? 2u : 0u) | (iz == 0 ? 4u : 0u) | 8u | (e == 0 ? 16u : 0u);
   }
 
   // ---- traversal.  Work units are dealt to blocks in XCD-contiguous ranges
@@ -388,10 +390,19 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
         if (carryS & (1u << jp)) ring[rr * SH::cS + ringq[jp]] = lds[carrysrc[jp]];
     }
     // shared nodes: x pairs, then y, then z
+    unsigned skip_seam = 0;  // axes on which this item holds the rank surface
+    if (I.skip_surface) {
+      const int row = it / I.nseg;
+      skip_seam = ((it % I.nseg) == 0 ? 1u : 0u) | ((row % G.my) == 0 ? 2u : 0u) |
+                  ((row / G.my) == 0 ? 4u : 0u);
+    }
 #pragma unroll
     for (int i = 0; i < NN; i++) {
       const unsigned f = nflag[i];
       if (!(f & 8u)) continue;
+      // rank-surface nodes (global coordinate 0 on an axis: the periodic
+      // wrap pairs) are left to the surface kernel when skipping
+      if (skip_seam && ((f & skip_seam & 6u) || ((f & 17u) == 17u && (skip_seam & 1u)))) continue;
       const bool cx = f & 1u, cy = f & 2u, cz = f & 4u;
       const unsigned p0 = np[i][0] & 0xffff, p1 = np[i][0] >> 16;
       const unsigned p2 = np[i][1] & 0xffff, p3 = np[i][1] >> 16;
