@@ -5,7 +5,10 @@ Workload (N=1): the plain step on a 1x1x1 periodic rank grid of 60x60x60
 elements of n=7 FP64 points (592.7 MB), integer-valued in [0,1023] from seed
 42, every message addressed to self.  One step = one share, i.e. one pass of
 the hot path over the field.  The field stays resident in HBM; the timed
-region holds only the share launches.
+region holds only the share launches: by default a HIP graph of ni = 10
+shares (the reference's inner loop, gpu/main.cc:176-183) replayed steps/10
+times, HIP events around every replay on the launch stream; --no-graph times
+every share launch on its own.
 
 value = algorithmic bytes per step / step time, with algorithmic bytes
 B_step = 16 * mx*my*mz*n^3 (the field read once and written once).
@@ -64,6 +67,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-kernel reference")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every share on its own instead of replaying a graph of --ni shares")
+    ap.add_argument("--ni", type=int, default=10, help="shares per captured graph")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
     ap.add_argument("--grids", type=int, default=256, help="grids in the batched leg")
@@ -263,17 +269,43 @@ def main():
         f.share(us[i % nbuf], stream=stream)
     torch.cuda.synchronize()
 
-    # Events around every launch on the launch stream: per-launch durations.
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    evs[0].record(stream)
-    for i in range(args.steps):
-        f.share(us[i % nbuf], stream=stream)
-        evs[i + 1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    per = [evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)]
+    if args.no_graph or nbuf > 1:
+        # Events around every launch on the launch stream: per-launch durations.
+        nrep, per_rep = args.steps, 1
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(nrep + 1)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        evs[0].record(stream)
+        for i in range(nrep):
+            f.share(us[i % nbuf], stream=stream)
+            evs[i + 1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    else:
+        # A graph of ni shares, replayed; events around every replay.
+        per_rep = max(1, args.ni)
+        nrep = max(1, args.steps // per_rep)
+        cap = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=cap):
+            for _ in range(per_rep):
+                f.share(us[0], stream=cap)
+        torch.cuda.synchronize()
+        graph.replay()  # one untimed replay (the first is excluded, as in the reference)
+        torch.cuda.synchronize()
+        # replay() launches on the current stream: the events go there too
+        rs = torch.cuda.current_stream()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(nrep + 1)]
+        t0 = time.perf_counter()
+        evs[0].record(rs)
+        for i in range(nrep):
+            graph.replay()
+            evs[i + 1].record(rs)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        args.steps = nrep * per_rep
+    per = [evs[i].elapsed_time(evs[i + 1]) / per_rep for i in range(nrep)]
     ms = sum(per) / len(per)
     gbs = b_step / (ms * 1e-3) / 1e9
     f.close()
@@ -306,7 +338,9 @@ def main():
                    "rank_grid": "1x1x1 periodic", "bytes_per_step": int(b_step),
                    "fields_rotated": nbuf,
                    "cache_resident": bool(nbuf * 8 * total < 512e6),
-                   "path": "fused" if args.path in (0, 1) else f"path{args.path}"},
+                   "path": "fused" if args.path in (0, 1) else f"path{args.path}",
+                   "timing": "per launch" if (args.no_graph or nbuf > 1)
+                   else f"graph of {args.ni} shares replayed"},
         "steps_per_s": round(1e3 / ms, 1),
         "step_ms_min": round(per_sorted[0], 5),
         "step_ms_p50": round(per_sorted[len(per) // 2], 5),
