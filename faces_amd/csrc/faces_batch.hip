@@ -264,11 +264,13 @@ struct faces_batch {
 
 namespace {
 
-// Grids the work-list kernel takes: n within its routines and a field its
-// 32-bit offsets reach.  The others step on their own plan.
+// Grids the work-list kernel takes: n within its routines and a field below
+// 64 MiB.  The others step on their own plan after the batched launch: large
+// grids run faster through the plain step's streaming kernel (and its 32-bit
+// offsets would not reach fields of 4 GB).
 bool batchable(int n, int mx, int my, int mz)
 {
-  return n >= 2 && n <= 12 && 8 * faces_field_doubles(n, mx, my, mz, 0) < 0xFFFFFFF0L;
+  return n >= 2 && n <= 12 && 8 * faces_field_doubles(n, mx, my, mz, 0) < 64L * 1024 * 1024;
 }
 
 int upload(faces_batch *b)

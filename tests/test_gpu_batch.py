@@ -234,3 +234,16 @@ def test_batch_invalid(torch_cuda):
     with pytest.raises(fa.FacesError):
         b.set_tuning(128, 0, 0)
     b.close()
+
+
+def test_batch_large_grid_on_own_plan(torch_cuda):
+    """A grid of 64 MiB or more steps on its own plan within the same share."""
+    print("case batch large grid", flush=True)
+    shapes = [(4, 3, 2, 2), (7, 30, 30, 30), (6, 2, 3, 4)]  # the middle one is 74 MB
+    fields = make_fields(torch_cuda, shapes, seed0=130)
+    b = Batch(shapes)
+    b.set_fields(fields)
+    b.share()
+    b.share()
+    assert_all_same(host(torch_cuda, fields), oracle_steps(shapes, fa.FILL_RAND, 130, 2), shapes)
+    b.close()
