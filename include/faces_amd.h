@@ -178,7 +178,8 @@ int faces_fill(double *u, int n, int mx, int my, int mz, long plane_stride, int 
 long faces_field_doubles(int n, int mx, int my, int mz, long plane_stride);
 
 /* ---- batched step: many independent grids, one launch -------------------- */
-/* No counterpart in the reference (north-star extension). */
+/* No counterpart in the reference, which steps one grid per process
+ * (gpu/main.cc:170-186). */
 typedef struct faces_batch faces_batch;
 
 /* Describes `ngrids` grids (dense fields).  boundary as above; every grid is
@@ -212,7 +213,8 @@ int faces_batch_tune(faces_batch *batch, int reps, void *stream, double *out);
 int faces_batch_info(const faces_batch *batch, double *bytes, long *items);
 
 /* ---- served step: continuous batching of requests ------------------------ */
-/* No counterpart in the reference (north-star extension). */
+/* No counterpart in the reference, which steps one grid per process
+ * (gpu/main.cc:170-186). */
 typedef struct faces_server faces_server;
 
 /* footprint_cap: bytes of request fields admitted at once; max_batch: most

@@ -85,7 +85,7 @@ long orc_field_doubles(int n, int mx, int my, int mz, long ps)
 }
 
 /* ------------------------------------------------------------------------ */
-/* seeded generators (not in the reference: the north-star inputs)           */
+/* seeded generators (not in the reference, which fills by position only)   */
 /* ------------------------------------------------------------------------ */
 
 /* splitmix64 finaliser of (seed xor idx*golden) -- Steele, Lea & Flood 2014 */
