@@ -83,6 +83,7 @@ _sig("faces_plan_set_tuning", _c_int, _vp, _c_int, _c_int, _c_int)
 _sig("faces_plan_set_variant", _c_int, _vp, _c_int, _c_int)
 _sig("faces_plan_set_option", _c_int, _vp, _c_int, _c_int)
 _sig("faces_plan_get_option", _c_int, _vp, _c_int, ctypes.POINTER(_c_int))
+_sig("faces_plan_tune", _c_int, _vp, _vp, _c_int, _vp, ctypes.POINTER(ctypes.c_double))
 _sig("faces_plan_neighbor", _c_int, _vp, _c_int, ctypes.POINTER(_c_int))
 _sig("faces_plan_message", _c_int, _vp, _c_int, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_long))
 _sig("faces_pack", _c_int, _vp, _vp, _vp)
@@ -189,6 +190,14 @@ class Faces:
         v = _c_int()
         _check(_lib.faces_plan_get_option(self._h, option, ctypes.byref(v)), "faces_plan_get_option")
         return v.value
+
+    def tune(self, u, reps=5, stream=None) -> dict:
+        """Pick the fastest launch options on this grid (shares u 1 + reps
+        times per trial: tune on a scratch copy)."""
+        out = (ctypes.c_double * 5)()
+        _check(_lib.faces_plan_tune(self._h, _addr(u), reps, _stream(stream), out), "faces_plan_tune")
+        return {"variant": int(out[0]), "walk": int(out[1]), "seg_len": int(out[2]),
+                "blocks_per_cu": int(out[3]), "us_per_share": out[4]}
 
     def neighbor(self, slot: int) -> int:
         r = _c_int()

@@ -557,7 +557,9 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
   p->xcd_remap = 1;
   p->variant = FACES_VARIANT_AUTO;
   p->lds_block = 256;
-  p->stream_bpc = 2;   // measured best on 60^3 n=7 and 32^3 n=8
+  // measured best: 2 blocks per CU (60^3 n=7, 64^3 n=8, 80^3 n=6, 128^3 n=4,
+  // 40^3 n=10 and n=12), 3 for n=5 (100^3: 558 us against 678 us)
+  p->stream_bpc = (n == 5) ? 3 : 2;
   p->stream_walk = 1;
   p->stream_rows = 4;
   p->items_bpc = 4;
@@ -720,6 +722,90 @@ int faces_plan_get_option(const faces_plan *p, int option, int *value)
     case FACES_OPT_STREAM_PLANES: *value = stream_big(p) ? 1 : 0; break;
     case FACES_OPT_ACTIVE_KERNEL: *value = active_kernel(p); break;
     default: return fail(FACES_ERR_ARG, "unknown option %d", option);
+  }
+  return FACES_OK;
+}
+
+int faces_plan_tune(faces_plan *p, double *u, int reps, void *stream, double *out)
+{
+  int st = check_plan(p, u);
+  if (st) return st;
+  if (reps < 1 || reps > 1000) return fail(FACES_ERR_ARG, "reps %d outside [1,1000]", reps);
+  struct Cand {
+    int variant, walk, rows, seg, bpc;
+  };
+  std::vector<Cand> cands;
+  const int seg0 = p->seg_len;
+  if (stream_ok(p)) {
+    // the three longest compiled segment lengths that divide mx
+    std::vector<int> segs;
+    for (int sl = 32; sl >= 1 && segs.size() < 3; sl--) {
+      p->seg_len = sl;
+      if (stream_seg(p) == sl) segs.push_back(sl);
+    }
+    for (int walk : {1, 3})
+      for (int rows : {2, 4}) {
+        if (walk == 1 && rows == 4) continue;
+        for (int sl : segs)
+          for (int bpc : {1, 2, 3, 4}) cands.push_back({FACES_VARIANT_STREAM, walk, walk == 3 ? rows : 1, sl, bpc});
+      }
+  }
+  p->seg_len = seg0;
+  if (items_ok(p))
+    for (int bpc : {2, 4}) cands.push_back({FACES_VARIANT_ITEMS, p->stream_walk, p->stream_rows, 0, bpc});
+  if (cands.empty()) return fail(FACES_ERR_STATE, "no tunable kernel for this grid");
+
+  const int v0 = p->variant, w0 = p->stream_walk, r0 = p->stream_rows, b0 = p->stream_bpc, i0 = p->items_bpc;
+  auto apply = [&](const Cand &c) {
+    p->variant = c.variant;
+    p->stream_walk = c.walk;
+    p->stream_rows = c.rows;
+    p->seg_len = c.seg;
+    if (c.variant == FACES_VARIANT_ITEMS) p->items_bpc = c.bpc;
+    else p->stream_bpc = c.bpc;
+  };
+  hipStream_t hs = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return fail(FACES_ERR_HIP, "hipEventCreate");
+  }
+  double best = 1e300;
+  Cand bestc = cands[0];
+  for (const Cand &c : cands) {
+    apply(c);
+    st = faces_share(p, u, 1, stream);  // warm-up
+    if (!st && hipEventRecord(e0, hs) != hipSuccess) st = fail(FACES_ERR_HIP, "hipEventRecord");
+    for (int r = 0; r < reps && !st; r++) st = faces_share(p, u, 1, stream);
+    float ms = 0.f;
+    if (!st && (hipEventRecord(e1, hs) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      st = fail(FACES_ERR_HIP, "timing events");
+    if (st) break;
+    const double us = 1e3 * ms / reps;
+    if (us < best) {
+      best = us;
+      bestc = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) {  // restore the settings the plan had
+    p->variant = v0;
+    p->stream_walk = w0;
+    p->stream_rows = r0;
+    p->seg_len = seg0;
+    p->stream_bpc = b0;
+    p->items_bpc = i0;
+    return st;
+  }
+  apply(bestc);
+  if (out) {
+    out[0] = bestc.variant;
+    out[1] = bestc.walk;
+    out[2] = bestc.variant == FACES_VARIANT_ITEMS ? batch_seg(p->n, p->mx, 64) : stream_seg(p);
+    out[3] = bestc.bpc;
+    out[4] = best;
   }
   return FACES_OK;
 }

@@ -149,6 +149,16 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
 int faces_plan_set_option(faces_plan *plan, int option, int value);
 int faces_plan_get_option(const faces_plan *plan, int option, int *value);
 
+/* Autotuner of the sum kernel (no counterpart in the reference, whose launch
+ * shape is fixed at build time, gpu/gpu.h:73-78): times the streaming kernel
+ * over walks, segment lengths and blocks per CU, and the work-list kernel over
+ * blocks per CU, on this plan's grid, and keeps the fastest.  Every trial
+ * shares `u` 1 + reps times (compute = 1): the field changes as after that
+ * many shares, so tune on a scratch copy.  Synchronises `stream`.  out (may
+ * be NULL), 5 doubles: variant, walk, elements per item, blocks per CU, µs per
+ * share of the choice. */
+int faces_plan_tune(faces_plan *plan, double *u, int reps, void *stream, double *out);
+
 /* Neighbour rank of message slot 0..25, or -1 for none (MPI_PROC_NULL) --
  * Faces::neighbor, gpu/v-26-messages/Faces.cc:92-103. */
 int faces_plan_neighbor(const faces_plan *plan, int slot, int *rank);

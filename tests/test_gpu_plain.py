@@ -462,3 +462,22 @@ def test_field_above_4gb_bitexact(torch_cuda):
     torch_cuda.cuda.empty_cache()
     oracle.share_sweep(u, *shape)
     assert np.array_equal(got, u)
+
+
+@pytest.mark.parametrize("shape", [(7, 12, 6, 4), (5, 8, 4, 3), (8, 7, 3, 2), (4, 16, 4, 2)])
+def test_plan_tune_then_bitexact(torch_cuda, shape):
+    """The autotuner picks a valid launch shape; a share with it is exact."""
+    n, mx, my, mz = shape
+    print("case plan tune", shape, flush=True)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    scratch = torch_cuda.empty(fa.field_doubles(*shape), dtype=torch_cuda.float64, device="cuda")
+    fa.fill(scratch, *shape, kind=fa.FILL_RAND, seed=9)
+    res = f.tune(scratch, reps=2)
+    assert res["variant"] in (fa.VARIANT_STREAM, fa.VARIANT_ITEMS) and res["us_per_share"] > 0
+    assert f.get_option(fa.OPT_ACTIVE_KERNEL) == res["variant"]
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=901)
+    d = to_dev(torch_cuda, u)
+    f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    assert_same(got, run_oracle(u, shape), shape)
