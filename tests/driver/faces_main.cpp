@@ -34,7 +34,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/faces_amd.h"
+#include "../../include/faces_amd.hpp"
 
 extern "C" {
 long orc_field_doubles(int n, int mx, int my, int mz, long ps);
@@ -53,13 +53,6 @@ int usage(const char *msg)
                "usage: faces_main [--boundary periodic|open] [--init int|rand|ref] [--seed S]\n"
                "                  [--rank R | --all-ranks] [--reference]  < 'lx ly lz mx my mz n nk nj ni'\n");
   return 2;
-}
-
-bool check(int st, const char *what)
-{
-  if (st == FACES_OK) return true;
-  std::fprintf(stderr, "faces_main: %s failed: %s (%s)\n", what, faces_strerror(st), faces_last_error());
-  return false;
 }
 
 // Self-halo bits of a rank in an open lx*ly*lz grid: bit 2*axis for a low
@@ -134,66 +127,57 @@ int main(int argc, char **argv)
   const bool compute = ni >= 0;
   const int nshare = ni < 0 ? -ni : ni;
 
-  const long nd = faces_field_doubles(n, mx, my, mz, 0);
-  if (nd <= 0) return usage("field size");
-  double *u = nullptr;
-  if (hipMalloc(&u, sizeof(double) * nd) != hipSuccess) {
-    std::fprintf(stderr, "faces_main: hipMalloc of %ld doubles failed\n", nd);
-    return 1;
-  }
-  std::vector<double> host(nd), want(nd);
-
   const int r0 = all_ranks ? 0 : rank, r1 = all_ranks ? nranks : rank + 1;
   int npass = 0, nexact = 0;
   double tsum = 0, tmin = 1e300, tmax = 0;
-  bool ok = true;
-  for (int r = r0; r < r1 && ok; r++) {
-    // the oracle's result for this rank: nshare shares of the initial field
-    orc_fill(want.data(), n, mx, my, mz, 0, kind, seed);
-    const int mask = boundary == FACES_BOUNDARY_OPEN ? neighbour_mask(r, lx, ly, lz) : 0;
-    for (int i = 0; i < nshare; i++) {
-      if (boundary == FACES_BOUNDARY_PERIODIC) orc_share_sweep(want.data(), n, mx, my, mz, 0, 1, compute);
-      else orc_share_staged(want.data(), n, mx, my, mz, 0, mask, compute);
-    }
-    double t = 0;
-    for (int k = 0; k < nk && ok; k++) {
-      if (r == r0) std::cout << (k + 1) << ": " << std::flush;
-      faces_plan *plan = nullptr;
-      ok = check(faces_plan_create(&plan, r, lx, ly, lz, mx, my, mz, n, boundary, 0), "faces_plan_create");
-      for (int j = 0; ok && j <= nj; j++) {
-        ok = check(faces_fill(u, n, mx, my, mz, 0, kind, seed, nullptr), "faces_fill") &&
-             hipDeviceSynchronize() == hipSuccess;
-        const auto t0 = std::chrono::steady_clock::now();
-        for (int i = 0; ok && i < nshare; i++) ok = check(faces_share(plan, u, compute, nullptr), "faces_share");
-        ok = ok && hipDeviceSynchronize() == hipSuccess;
-        const auto t1 = std::chrono::steady_clock::now();
-        if (j > 0) t += std::chrono::duration<double>(t1 - t0).count();
-      }
-      if (plan) faces_plan_destroy(plan);
-    }
-    if (!ok) break;
-    if (nk == 0) {  // nothing stepped: the field is the initial one
+  try {
+    faces_amd::Faces::Double6D u(n, mx, my, mz);  // gpu/main.cc:169
+    const long nd = u.size();
+    std::vector<double> host, want(nd);
+    for (int r = r0; r < r1; r++) {
+      // the oracle's result for this rank: nshare shares of the initial field
       orc_fill(want.data(), n, mx, my, mz, 0, kind, seed);
-      faces_fill(u, n, mx, my, mz, 0, kind, seed, nullptr);
-      (void)hipDeviceSynchronize();
+      const int mask = boundary == FACES_BOUNDARY_OPEN ? neighbour_mask(r, lx, ly, lz) : 0;
+      for (int i = 0; i < nshare; i++) {
+        if (boundary == FACES_BOUNDARY_PERIODIC) orc_share_sweep(want.data(), n, mx, my, mz, 0, 1, compute);
+        else orc_share_staged(want.data(), n, mx, my, mz, 0, mask, compute);
+      }
+      // gpu/main.cc:172-184: nk plans, each nj+1 initialisations (the first
+      // untimed) of nshare shares
+      double t = 0;
+      for (int k = 0; k < nk; k++) {
+        if (r == r0) std::cout << (k + 1) << ": " << std::flush;
+        faces_amd::Faces faces(r, lx, ly, lz, mx, my, mz, n, boundary);
+        for (int j = 0; j <= nj; j++) {
+          u.fill(kind, seed);
+          faces_amd::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+          const auto t0 = std::chrono::steady_clock::now();
+          for (int i = 0; i < nshare; i++) faces.share(u, compute);
+          faces_amd::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+          const auto t1 = std::chrono::steady_clock::now();
+          if (j > 0) t += std::chrono::duration<double>(t1 - t0).count();
+        }
+      }
+      if (nk == 0) {  // nothing stepped: the field is the initial one
+        orc_fill(want.data(), n, mx, my, mz, 0, kind, seed);
+        u.fill(kind, seed);
+        faces_amd::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      }
+      u.copy(host);
+      double cmp[4];
+      orc_compare(host.data(), want.data(), n, mx, my, mz, 0, cmp);
+      // gpu/main.cc:65-80: pass when the largest |u-v|/(|u|+|v|) is below 1e-15
+      if (cmp[0] < 1e-15) npass++;
+      else std::cout << r << " FAIL " << cmp[0] << " (index " << (long)cmp[1] << ")" << std::endl;
+      if (cmp[2] == 0) nexact++;
+      tsum += t;
+      tmin = std::min(tmin, t);
+      tmax = std::max(tmax, t);
     }
-    if (hipMemcpy(host.data(), u, sizeof(double) * nd, hipMemcpyDeviceToHost) != hipSuccess) {
-      std::fprintf(stderr, "faces_main: copy of the field failed\n");
-      ok = false;
-      break;
-    }
-    double cmp[4];
-    orc_compare(host.data(), want.data(), n, mx, my, mz, 0, cmp);
-    // gpu/main.cc:65-80: pass when the largest |u-v|/(|u|+|v|) is below 1e-15
-    if (cmp[0] < 1e-15) npass++;
-    else std::cout << r << " FAIL " << cmp[0] << " (index " << (long)cmp[1] << ")" << std::endl;
-    if (cmp[2] == 0) nexact++;
-    tsum += t;
-    tmin = std::min(tmin, t);
-    tmax = std::max(tmax, t);
+  } catch (const faces_amd::Error &e) {
+    std::fprintf(stderr, "faces_main: %s\n", e.what());
+    return 1;
   }
-  (void)hipFree(u);
-  if (!ok) return 1;
   const int nchecked = r1 - r0;
   std::cout << npass << " tasks passed correctness check" << std::endl;
   std::cout << "time " << tsum / nchecked << " avg " << tmin << " min " << tmax << " max" << std::endl;
