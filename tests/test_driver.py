@@ -73,3 +73,27 @@ def test_driver_passes(stdin, args, npass):
     assert "bit-exact" in err
     m = re.search(r"(\d+) of (\d+) ranks bit-exact", err)
     assert m and m.group(1) == m.group(2), err
+
+
+@pytest.mark.gpu
+@needs_driver
+def test_reference_run_line_a():
+    """The reference's run line A (gpu/frontier-run-all-1.sh:15) on one rank in
+    the reference's own mode: open boundary, positional init, 11,000 shares."""
+    rc, out, err = run("1 1 1 15 14 13 12 10 10 100", "--reference", timeout=300)
+    assert rc == 0, (out, err)
+    check_output(out, (1, 1, 1), (15, 14, 13), 12, 10, 10, 100, 1)
+    assert "1 of 1 ranks bit-exact" in err
+
+
+@pytest.mark.gpu
+@pytest.mark.slow
+@needs_driver
+def test_reference_run_line_b():
+    """The reference's run line B (gpu/frontier-run-all-1.sh:17: 95x94x93
+    elements, n=12, an 11.5 GB field, 3 plans x 2 inits x 10 shares) on one
+    rank in the reference's own mode."""
+    rc, out, err = run("1 1 1 95 94 93 12 3 1 10", "--reference", timeout=600)
+    assert rc == 0, (out, err)
+    check_output(out, (1, 1, 1), (95, 94, 93), 12, 3, 1, 10, 1)
+    assert "1 of 1 ranks bit-exact" in err
