@@ -169,7 +169,23 @@ def served_leg(torch, fa, nreq, args_nstreams=2):
     warm = srv.run_trace([0.0, 10.0, 20.0], shapes[:3], fields[:3])
     del warm
     lat, st = srv.run_trace(arrivals, shapes, fields)
+    # capacity: every request queued at once
+    _, cap_st = srv.run_trace([0.0] * nreq, shapes, fields)
     srv.close()
+    # the same requests one plain launch each, in arrival order
+    plans = {}
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sh, d in zip(shapes, fields):
+        if sh not in plans:
+            n, mx, my, mz = sh
+            plans[sh] = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+        plans[sh].share(d, stream=stream)
+    torch.cuda.synchronize()
+    per_req_s = time.perf_counter() - t0
+    for f in plans.values():
+        f.close()
     nbytes = sum(16.0 * n ** 3 * mx * my * mz for n, mx, my, mz in shapes)
     nlarge = sum(1 for sh in shapes if sh == (7, 48, 48, 48))
     del fields
@@ -182,7 +198,10 @@ def served_leg(torch, fa, nreq, args_nstreams=2):
             "p99_us": round(st["p99_us"], 1), "max_us": round(st["max_us"], 1),
             "launches": int(st["launches"]), "footprint_cap_bytes": cap,
             "streams": args_nstreams,
-            "GBs": round(nbytes / st["wall_s"] / 1e9, 2)}
+            "GBs": round(nbytes / st["wall_s"] / 1e9, 2),
+            "capacity_req_per_s": round(cap_st["req_per_s"], 1),
+            "per_request_launch_req_per_s": round(nreq / per_req_s, 1),
+            "capacity_vs_per_request_launch": round(cap_st["req_per_s"] * per_req_s / nreq, 3)}
 
 
 def copy_reference_gbs(torch, nbytes, reps=20):
