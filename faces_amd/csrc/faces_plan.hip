@@ -230,10 +230,15 @@ constexpr int kStreamMaxSlots = 12;
 
 // Segment lengths the streaming kernel is compiled for, per n: 1-4 for all,
 // and longer ones where n is small (a 4-element item of n=4 is 256 points).
+// S = 5 fills the 256-lane slots of n = 5-8 items closer to whole than S = 4
+// (n=5: 750 of 768 footprint slots against 605; n=7: 1974 of 2048 against
+// 1589 of 1792), S = 10 those of n = 5 (1475 of 1536).
 bool stream_case(int n, int S)
 {
   switch (S) {
     case 1: case 2: case 3: case 4: return true;
+    case 5: return n >= 5 && n <= 8;
+    case 10: return n == 5;
     case 6: case 8: return n <= 6;
     case 16: return n <= 4;
     case 32: return n == 2;
@@ -247,7 +252,7 @@ bool stream_case(int n, int S)
 int stream_seg(const faces_plan *p)
 {
   if (p->n < 2 || p->n > 12) return 0;
-  static const int kDefault[13] = {0, 0, 32, 16, 16, 8, 4, 4, 4, 3, 2, 1, 1};
+  static const int kDefault[13] = {0, 0, 32, 16, 16, 10, 5, 4, 4, 3, 2, 1, 1};
   const int want = p->seg_len > 0 ? p->seg_len : kDefault[p->n];
   for (int s = std::min(want, 32); s >= 1; s--) {
     if (!stream_case(p->n, s) || p->mx % s) continue;
@@ -317,6 +322,12 @@ int launch_stream_n(const faces_plan *p, const Geom &G, int S, bool skip, hipStr
     case 3: return launch_stream_ns<N, 3>(p, G, skip, st);
     case 4: return launch_stream_ns<N, 4>(p, G, skip, st);
     default: break;
+  }
+  if constexpr (N >= 5 && N <= 8) {
+    if (S == 5) return launch_stream_ns<N, 5>(p, G, skip, st);
+  }
+  if constexpr (N == 5) {
+    if (S == 10) return launch_stream_ns<N, 10>(p, G, skip, st);
   }
   if constexpr (N <= 6) {
     if (S == 6) return launch_stream_ns<N, 6>(p, G, skip, st);
@@ -558,8 +569,10 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
   p->variant = FACES_VARIANT_AUTO;
   p->lds_block = 256;
   // measured best: 2 blocks per CU (60^3 n=7, 64^3 n=8, 80^3 n=6, 128^3 n=4,
-  // 40^3 n=10 and n=12), 3 for n=5 (100^3: 558 us against 678 us)
-  p->stream_bpc = (n == 5) ? 3 : 2;
+  // 40^3 n=10 and n=12, 80^3 and 100^3 n=5 with 10-element items), 3 for the
+  // short items of n=5 (S <= 5, three footprint slots per thread: 100^3 with
+  // S=4 560 us against 677 us)
+  p->stream_bpc = (n == 5 && stream_seg(p) > 0 && stream_slots(n, stream_seg(p)) <= 3) ? 3 : 2;
   p->stream_walk = 1;
   p->stream_rows = 4;
   p->items_bpc = 4;

@@ -344,10 +344,13 @@ def test_items_padded_plane_stride(torch_cuda, ps_pad):
 
 @pytest.mark.parametrize("shape,seg", [((4, 32, 2, 2), 16), ((2, 64, 2, 1), 32), ((3, 16, 3, 2), 16),
                                        ((6, 16, 2, 2), 8), ((5, 12, 3, 1), 6), ((6, 24, 1, 2), 6),
-                                       ((5, 8, 2, 2), 8), ((4, 16, 1, 1), 8)])
+                                       ((5, 8, 2, 2), 8), ((4, 16, 1, 1), 8), ((5, 10, 3, 2), 5),
+                                       ((5, 20, 2, 2), 10), ((6, 10, 2, 2), 5), ((7, 15, 2, 3), 5),
+                                       ((8, 5, 3, 2), 5)])
 @pytest.mark.parametrize("walk", [0, 1, 2, 3])
 def test_stream_long_segments_bitexact(torch_cuda, shape, seg, walk):
-    """Long items of the streaming kernel for small n."""
+    """Long items of the streaming kernel for small n, and the 5- and
+    10-element items of n = 5-8."""
     n, mx, my, mz = shape
     print("case stream long segments", shape, seg, walk, flush=True)
     u = oracle.new_field(*shape, fa.FILL_RAND, seed=406)
@@ -420,7 +423,8 @@ def test_six_c2_size_integer(torch_cuda):
 
 @pytest.mark.parametrize("walk,rows", [(0, 1), (1, 1), (2, 1), (3, 2)])
 @pytest.mark.parametrize("shape,seg", [((7, 12, 3, 4), 4), ((4, 32, 2, 3), 16), ((12, 3, 2, 2), 1),
-                                       ((5, 6, 1, 1), 3), ((8, 4, 3, 1), 2)])
+                                       ((5, 6, 1, 1), 3), ((8, 4, 3, 1), 2), ((7, 10, 3, 2), 5),
+                                       ((5, 10, 2, 3), 10)])
 def test_stream_per_plane_bitexact(torch_cuda, walk, rows, shape, seg):
     """The per-plane descriptors that fields above 4 GB need, forced on small
     fields (every walk, wrap planes included)."""
