@@ -493,12 +493,14 @@ int faces_batch_info(const faces_batch *b, double *bytes, long *items)
 int faces_batch_tune(faces_batch *b, int reps, void *stream, double *out)
 {
   if (!b || !out) return fail(FACES_ERR_ARG, "null argument");
-  if (reps < 1) return fail(FACES_ERR_ARG, "reps must be positive");
+  if (reps < 1 || reps > 1000) return fail(FACES_ERR_ARG, "reps %d outside [1,1000]", reps);
   if (!fields_bound(b)) return fail(FACES_ERR_STATE, "fields not bound (faces_batch_set_fields)");
   hipStream_t st = (hipStream_t)stream;
-  hipEvent_t e0, e1;
-  FACES_HIP(hipEventCreate(&e0));
-  FACES_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return fail(FACES_ERR_HIP, "hipEventCreate");
+  }
   double best = 1e30;
   int best_seg = b->seg_cap, best_bpc = b->bpc;
   int rc = FACES_OK;

@@ -210,6 +210,9 @@ void faces_server::run()
       if (batch.empty()) break;  // the head waits for footprint to free up
       const int st = launch(s, batch);
       if (st) {
+        // part of the batch may already be enqueued: let it finish before
+        // the caller learns of the failure (and may free its fields)
+        (void)hipStreamSynchronize(s.stream);
         for (Request *r : batch) {
           r->status = st;
           r->done = true;
@@ -332,14 +335,22 @@ int faces_server_wait(faces_server *s, uint64_t ticket, double timeout_s, double
 {
   if (!s) return fail(FACES_ERR_ARG, "null server");
   std::unique_lock<std::mutex> lk(s->mu);
-  auto it = s->reqs.find(ticket);
-  if (it == s->reqs.end()) return fail(FACES_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
-  Request *r = it->second;
   const auto deadline = Clock::now() + std::chrono::duration<double>(timeout_s > 0 ? timeout_s : 1e9);
-  while (!r->done) {
-    if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && !r->done)
-      return fail(FACES_ERR_TIMEOUT, "request %llu not done", (unsigned long long)ticket);
+  // The ticket is looked up again after every wake-up: another thread may
+  // have waited for it (and released it) meanwhile.
+  auto it = s->reqs.find(ticket);
+  while (it != s->reqs.end() && !it->second->done) {
+    if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
+      it = s->reqs.find(ticket);
+      if (it != s->reqs.end() && !it->second->done)
+        return fail(FACES_ERR_TIMEOUT, "request %llu not done", (unsigned long long)ticket);
+      break;
+    }
+    it = s->reqs.find(ticket);
   }
+  if (it == s->reqs.end())
+    return fail(FACES_ERR_ARG, "unknown ticket %llu (never issued, or already waited for)", (unsigned long long)ticket);
+  Request *r = it->second;
   const int st = r->status;
   if (latency_us) *latency_us = r->t_done - r->t_arrival;
   s->reqs.erase(it);

@@ -237,7 +237,10 @@ int faces_server_destroy(faces_server *srv);
 int faces_server_submit(faces_server *srv, int n, int mx, int my, int mz, int boundary, int compute,
                         double *u, uint64_t *ticket);
 
-/* Waits for a request; latency_us = completion - submission. */
+/* Waits for a request and releases its ticket: a ticket is waited for once
+ * (again, or never issued: FACES_ERR_ARG); FACES_ERR_TIMEOUT after timeout_s
+ * seconds (<= 0: no limit) leaves it pending.  latency_us = completion -
+ * submission.  Submit and wait may be called from any threads. */
 int faces_server_wait(faces_server *srv, uint64_t ticket, double timeout_s, double *latency_us);
 
 /* Replays an arrival trace: request i is submitted at arrival_us[i] after the

@@ -140,6 +140,44 @@ def test_wait_unknown_ticket(torch_cuda):
     srv.close()
 
 
+def test_ticket_waited_once(torch_cuda):
+    """A ticket is released by its wait: waiting again is an error, and of
+    two threads waiting on one ticket exactly one gets the result."""
+    import threading
+    print("case serve ticket waited once", flush=True)
+    shape = (5, 3, 3, 3)
+    srv = Server(footprint_cap=1e9, max_batch=4, nstreams=1)
+    d = field(torch_cuda, shape, 700)
+    torch_cuda.cuda.synchronize()
+    t = srv.submit(d, *shape)
+    srv.wait(t)
+    with pytest.raises(fa.FacesError) as e:
+        srv.wait(t, timeout_s=0.1)
+    assert e.value.status == fa.ERR_ARG
+    fields = [field(torch_cuda, shape, 710 + i) for i in range(8)]
+    torch_cuda.cuda.synchronize()
+    for i, d in enumerate(fields):
+        t = srv.submit(d, *shape)
+        outcome = []
+
+        def waiter():
+            try:
+                srv.wait(t, timeout_s=30.0)
+                outcome.append("ok")
+            except fa.FacesError as x:
+                outcome.append(x.status)
+
+        th = [threading.Thread(target=waiter) for _ in range(2)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        assert sorted(outcome, key=str) == sorted(["ok", fa.ERR_ARG], key=str), outcome
+        r = oracle.compare(d.cpu().numpy(), oracle_one(shape, 710 + i), *shape)
+        assert r["ndiff"] == 0, (i, r)
+    srv.close()
+
+
 def test_large_requests_launch_alone(torch_cuda):
     """A request of 64 MiB or more gets a launch of its own; the small ones
     around it are stepped in other launches, all bit-exact."""
