@@ -64,7 +64,8 @@ struct faces_plan {
   int lds_block; // threads per block of the LDS-staged kernel
   int stream_bpc; // persistent blocks per CU of the streaming kernel
   int stream_walk;   // streaming kernel traversal: 0 items, 1 x walk, 2 y walk, 3 row groups
-  int stream_rows;   // walk 3: rows per group (the largest divisor of my not above it)
+  int stream_rows;   // walk 3: rows per group; walk 2: rows per column piece (the largest
+                     // divisor of my not above it); 0 = default per walk
   int items_bpc;     // work-list item kernel: persistent blocks per CU
   int stream_planes; // 1: per-plane descriptors even below 4 GB
   // message arena: send slots then receive slots
@@ -289,14 +290,23 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, bool skip, hipStream_t 
   I.walk = p->stream_walk;
   I.skip_surface = skip ? 1 : 0;
   I.rows = 1;
-  if (I.walk == 3)
-    for (int r = std::max(1, std::min(p->stream_rows, p->my)); r >= 1; r--)
-      if (p->my % r == 0) {
-        I.rows = r;
-        break;
-      }
+  if (I.walk == 2 || I.walk == 3) {
+    auto divisor_upto = [&](int want) {
+      for (int r = std::max(1, std::min(want, p->my)); r > 1; r--)
+        if (p->my % r == 0) return r;
+      return 1;
+    };
+    I.rows = divisor_upto(p->stream_rows > 0 ? p->stream_rows : (I.walk == 2) ? 20 : 4);
+    // Default column pieces of walk 2: up to 20 rows (measured best on
+    // 60^3 n=7 and n=8, 64^3 n=8, 40^3 n=12), shorter while they would give
+    // fewer than 4 pieces per block (40^3 n=10: 10 rows 231 us, 20 rows 240 us).
+    if (I.walk == 2 && p->stream_rows == 0 && p->stream_bpc > 0) {
+      const long nbk = (long)device_cus() * p->stream_bpc;
+      while (I.rows > 1 && (long)p->mz * I.nseg * (p->my / I.rows) < 4 * nbk) I.rows = divisor_upto(I.rows - 1);
+    }
+  }
   const long units = (I.walk == 1)   ? (long)p->my * p->mz
-                     : (I.walk == 2) ? (long)p->mz * I.nseg
+                     : (I.walk == 2) ? (long)p->mz * I.nseg * (p->my / I.rows)
                      : (I.walk == 3) ? (long)p->mz * (p->my / I.rows)
                                      : I.nitems;
   // stream_bpc = 0: one unit per block (no persistence)
@@ -573,8 +583,11 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
   // short items of n=5 (S <= 5, three footprint slots per thread: 100^3 with
   // S=4 560 us against 677 us)
   p->stream_bpc = (n == 5 && stream_seg(p) > 0 && stream_slots(n, stream_seg(p)) <= 3) ? 3 : 2;
-  p->stream_walk = 1;
-  p->stream_rows = 4;
+  // column pieces walked in y for n >= 7 (60^3 n=7 275 us against 282-288 us
+  // for rows walked in x, 64^3 n=8 476 against 511, 40^3 n=12 457 against
+  // 472); rows walked in x for n <= 6 (128^3 n=4 515 against 532)
+  p->stream_walk = (n >= 7) ? 2 : 1;
+  p->stream_rows = 0;
   p->items_bpc = 4;
   p->stream_planes = 0;
 
@@ -710,7 +723,7 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       p->stream_walk = value;
       return FACES_OK;
     case FACES_OPT_STREAM_ROWS:
-      if (value < 1 || value > 64) return fail(FACES_ERR_ARG, "rows per group %d outside [1,64]", value);
+      if (value < 0 || value > 64) return fail(FACES_ERR_ARG, "rows per group %d outside [0,64]", value);
       p->stream_rows = value;
       return FACES_OK;
     case FACES_OPT_STREAM_PLANES:
@@ -756,12 +769,12 @@ int faces_plan_tune(faces_plan *p, double *u, int reps, void *stream, double *ou
       p->seg_len = sl;
       if (stream_seg(p) == sl) segs.push_back(sl);
     }
-    for (int walk : {1, 3})
-      for (int rows : {2, 4}) {
-        if (walk == 1 && rows == 4) continue;
-        for (int sl : segs)
-          for (int bpc : {1, 2, 3, 4}) cands.push_back({FACES_VARIANT_STREAM, walk, walk == 3 ? rows : 1, sl, bpc});
-      }
+    // (walk, rows): rows in x; column pieces in y of the default length and
+    // of 10 rows; groups of 2 and 4 rows
+    const int wr[5][2] = {{1, 1}, {2, 0}, {2, 10}, {3, 2}, {3, 4}};
+    for (const auto &w : wr)
+      for (int sl : segs)
+        for (int bpc : {1, 2, 3, 4}) cands.push_back({FACES_VARIANT_STREAM, w[0], w[1], sl, bpc});
   }
   p->seg_len = seg0;
   if (items_ok(p))

@@ -61,7 +61,7 @@ struct StreamItems {
   int xcd_remap;        // 1: contiguous ranges per XCD
   int walk;             // 0 items, 1 rows in x (seam carried), 2 columns in y (y rows carried),
                         // 3 groups of `rows` rows, segment by segment (seam and y rows carried)
-  int rows;             // walk 3: rows per group (divides my)
+  int rows;             // walk 3: rows per group; walk 2: rows per column piece (divides my)
   int skip_surface;     // 1: sum only nodes off the rank surface (open ranks:
                         // the surface is summed with the received copies)
 };
@@ -221,13 +221,14 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
 
   // ---- traversal.  Work units are dealt to blocks in XCD-contiguous ranges
   // (block b runs on XCD b % 8) or in one global order.  A unit is one item
-  // (walk 0), an element row walked in x (walk 1), a column of rows at
-  // fixed (jz, segment) walked in y (walk 2), or a group of R rows walked
+  // (walk 0), an element row walked in x (walk 1), L consecutive rows of
+  // the column at fixed (jz, segment) walked in y (walk 2), or a group of R rows walked
   // segment by segment, the R rows of a segment in turn (walk 3).  Items that
   // share memory sectors then run back to back on one CU, whose L2 merges
   // their writes, and carried values are not fetched twice.
+  const int L = (I.walk == 2) ? I.rows : 1;  // walk 2: rows per column piece
   const int units = (I.walk == 1) ? I.my * I.mz
-                    : (I.walk == 2) ? I.mz * I.nseg
+                    : (I.walk == 2) ? I.mz * I.nseg * (I.my / L)
                     : (I.walk == 3) ? I.mz * (I.my / R)
                                     : I.nitems;
   int start, cnt, k, nbx;
@@ -245,12 +246,14 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     nbx = gridDim.x;
   }
   if (k >= cnt) return;
-  const int nsegu = zig ? R * I.nseg : (I.walk == 2) ? G.my : 1;  // items per unit
+  const int nsegu = zig ? R * I.nseg : (I.walk == 2) ? L : 1;  // items per unit
   int sg = 0;
   auto item_of = [&](int kk, int ss) {
     const int c = start + kk;
     if (zig) return (c * R + ss % R) * I.nseg + ss / R;  // row c*R + r, segment ss/R
-    if (I.walk == 2) return ((c / I.nseg) * G.my + ss) * I.nseg + (c % I.nseg);
+    // walk 2: c = (jz * (my / L) + piece) * nseg + segment; its rows start at
+    // jz * my + piece * L = (c / nseg) * L
+    if (I.walk == 2) return ((c / I.nseg) * L + ss) * I.nseg + (c % I.nseg);
     return c;
   };
   double *const ring = lds + SH::P;  // saved seam columns, one per row of the group

@@ -126,12 +126,17 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *                            work unit per block)
  *   FACES_OPT_STREAM_WALK    STREAM kernel traversal: 0 = independent items,
  *                            1 = a block walks element rows in x carrying the
- *                            seam column, 2 = a block walks columns of rows in
- *                            y carrying the y-partner rows, 3 = a block walks
- *                            a group of rows segment by segment, carrying both
+ *                            seam column (default for n <= 6), 2 = a block
+ *                            walks pieces of columns of rows in y carrying the
+ *                            y-partner rows (default for n >= 7), 3 = a block
+ *                            walks a group of rows segment by segment,
+ *                            carrying both
  *   FACES_OPT_ACTIVE_KERNEL  (get only) the variant faces_share launches
- *   FACES_OPT_STREAM_ROWS    STREAM walk 3: rows per group (the largest divisor
- *                            of my not above the value is used)
+ *   FACES_OPT_STREAM_ROWS    STREAM walk 3: rows per group; walk 2: rows per
+ *                            piece of a column (the largest divisor of my not
+ *                            above the value is used); 0 (default) = 4 rows
+ *                            per group, and column pieces of up to 20 rows,
+ *                            shorter while there are fewer than 4 per block
  *   FACES_OPT_STREAM_PLANES  STREAM kernel addressing: 1 = through one
  *                            descriptor per jz plane (what fields of 4 GB and
  *                            more use; set, it applies to any field); get:

@@ -98,7 +98,8 @@ def test_periodic_variants_bitexact(torch_cuda, shape, variant):
     assert_same(run_gpu(torch_cuda, u, shape, variant=variant), run_oracle(u, shape), shape)
 
 
-@pytest.mark.parametrize("walk,rows", [(0, 1), (1, 1), (2, 1), (3, 1), (3, 2), (3, 3), (3, 4)])
+@pytest.mark.parametrize("walk,rows", [(0, 1), (1, 1), (2, 1), (2, 2), (2, 3), (2, 64), (3, 1), (3, 2),
+                                       (3, 3), (3, 4)])
 @pytest.mark.parametrize("seg,bpc,xcd", [(1, 0, 1), (2, 1, 0), (3, 2, 1), (4, 3, 1)])
 @pytest.mark.parametrize("shape", [(7, 12, 6, 3), (8, 4, 3, 5), (5, 6, 1, 2), (4, 3, 4, 1),
                                    (6, 2, 2, 2)])
@@ -421,7 +422,7 @@ def test_six_c2_size_integer(torch_cuda):
 
 # --------------------------------- per-plane addressing (fields of 4 GB and more)
 
-@pytest.mark.parametrize("walk,rows", [(0, 1), (1, 1), (2, 1), (3, 2)])
+@pytest.mark.parametrize("walk,rows", [(0, 1), (1, 1), (2, 1), (2, 3), (3, 2)])
 @pytest.mark.parametrize("shape,seg", [((7, 12, 3, 4), 4), ((4, 32, 2, 3), 16), ((12, 3, 2, 2), 1),
                                        ((5, 6, 1, 1), 3), ((8, 4, 3, 1), 2), ((7, 10, 3, 2), 5),
                                        ((5, 10, 2, 3), 10)])
