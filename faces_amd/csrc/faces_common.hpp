@@ -20,6 +20,7 @@ int hip_fail(hipError_t e, const char *what);
 
 // Number of compute units of the current device (cached per device).
 int device_cus();
+int device_lds_per_cu();
 
 }  // namespace faces
 

@@ -47,6 +47,22 @@ int device_cus()
   return cached[dev];
 }
 
+// LDS bytes per CU (160 KiB on gfx950).
+int device_lds_per_cu()
+{
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 160 * 1024;
+  if (cached[dev] == 0) {
+    int b = 0;
+    if (hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        b <= 0)
+      b = 160 * 1024;
+    cached[dev] = b;
+  }
+  return cached[dev];
+}
+
 }  // namespace faces
 
 using namespace faces;
@@ -296,12 +312,17 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, bool skip, hipStream_t 
         if (p->my % r == 0) return r;
       return 1;
     };
-    I.rows = divisor_upto(p->stream_rows > 0 ? p->stream_rows : (I.walk == 2) ? 20 : 4);
-    // Default column pieces of walk 2: up to 20 rows (measured best on
-    // 60^3 n=7 and n=8, 64^3 n=8, 40^3 n=12), shorter while they would give
-    // fewer than 4 pieces per block (40^3 n=10: 10 rows 231 us, 20 rows 240 us).
-    if (I.walk == 2 && p->stream_rows == 0 && p->stream_bpc > 0) {
-      const long nbk = (long)device_cus() * p->stream_bpc;
+    I.rows = divisor_upto(p->stream_rows > 0   ? p->stream_rows
+                          : (I.walk == 3)      ? 4
+                          : (p->stream_bpc < 0) ? 10
+                                                : 20);
+    // Default column pieces of walk 2: up to 20 rows for persistent blocks
+    // (measured best on 60^3 n=7), up to 10 for one piece per block (60^3
+    // n=8 385 us against 396, 40^3 n=12 395 against 409), shorter while they
+    // would give fewer than 4 pieces per resident block (40^3 n=10, 2 blocks
+    // per CU: 10 rows 231 us, 20 rows 240 us).
+    if (I.walk == 2 && p->stream_rows == 0 && p->stream_bpc != 0) {
+      const long nbk = (long)device_cus() * (p->stream_bpc > 0 ? p->stream_bpc : -p->stream_bpc);
       while (I.rows > 1 && (long)p->mz * I.nseg * (p->my / I.rows) < 4 * nbk) I.rows = divisor_upto(I.rows - 1);
     }
   }
@@ -309,16 +330,23 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, bool skip, hipStream_t 
                      : (I.walk == 2) ? (long)p->mz * I.nseg * (p->my / I.rows)
                      : (I.walk == 3) ? (long)p->mz * (p->my / I.rows)
                                      : I.nitems;
-  // stream_bpc = 0: one unit per block (no persistence)
+  // stream_bpc > 0: that many persistent blocks per CU; <= 0: one unit per
+  // block, and for -c at most c blocks resident per CU, each block reserving
+  // just over 1/(c+1) of the CU's LDS (the kernel uses only its footprint)
   long nb = p->stream_bpc > 0 ? (long)device_cus() * p->stream_bpc : units;
   if (nb > units) nb = units;
   if (I.xcd_remap) nb = (nb + 7) / 8 * 8;
   const size_t lds = stream_lds_doubles<N, S>(I.walk, I.rows) * sizeof(double);
   if (lds > 64 * 1024) return fail(FACES_ERR_ARG, "streaming kernel LDS %zu B above 64 KiB", lds);
+  size_t lds_launch = lds;
+  if (p->stream_bpc < 0) {
+    const size_t share = (size_t)device_lds_per_cu() / (size_t)(1 - p->stream_bpc) + 1024;
+    lds_launch = std::max(lds, std::min(share, (size_t)64 * 1024));
+  }
   if (big)
-    k_stream<N, S, true><<<dim3((unsigned)nb), dim3(kStreamThreads), lds, st>>>(G, I);
+    k_stream<N, S, true><<<dim3((unsigned)nb), dim3(kStreamThreads), lds_launch, st>>>(G, I);
   else
-    k_stream<N, S, false><<<dim3((unsigned)nb), dim3(kStreamThreads), lds, st>>>(G, I);
+    k_stream<N, S, false><<<dim3((unsigned)nb), dim3(kStreamThreads), lds_launch, st>>>(G, I);
   FACES_HIP(hipGetLastError());
   return FACES_OK;
 }
@@ -578,15 +606,24 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
   p->xcd_remap = 1;
   p->variant = FACES_VARIANT_AUTO;
   p->lds_block = 256;
-  // measured best: 2 blocks per CU (60^3 n=7, 64^3 n=8, 80^3 n=6, 128^3 n=4,
-  // 40^3 n=10 and n=12, 80^3 and 100^3 n=5 with 10-element items), 3 for the
-  // short items of n=5 (S <= 5, three footprint slots per thread: 100^3 with
-  // S=4 560 us against 677 us)
-  p->stream_bpc = (n == 5 && stream_seg(p) > 0 && stream_slots(n, stream_seg(p)) <= 3) ? 3 : 2;
+  // Blocks: 2 persistent blocks per CU for n = 6-7 (60^3 n=7 276 us against
+  // 281-284 us for one unit per block); one unit (column piece or row) per
+  // block, at most 2 resident per CU, for n >= 8 (64^3 n=8 448-450 us
+  // against 480, 40^3 n=12 395 against 460, 48^3 n=9 287 against 305), at
+  // most 3 for n <= 5 (128^3 n=4 478 against 518, 100^3 n=5 503 against 528).
+  // Fields under 256 MiB keep 2 persistent blocks for n >= 8 (32^3 n=8, six
+  // fields in turn: 66 us against 71).
+  const bool small = 8.0 * faces_field_doubles(n, mx, my, mz, plane_stride) < 256.0 * 1024 * 1024;
+  p->stream_bpc = (n >= 8) ? (small ? 2 : -2) : (n >= 6) ? 2 : -3;
   // column pieces walked in y for n >= 7 (60^3 n=7 275 us against 282-288 us
   // for rows walked in x, 64^3 n=8 476 against 511, 40^3 n=12 457 against
-  // 472); rows walked in x for n <= 6 (128^3 n=4 515 against 532)
-  p->stream_walk = (n >= 7) ? 2 : 1;
+  // 472) where my has a divisor from 4 to 20 to cut the columns by (95x94x93
+  // n=12, my = 2*47: 2-row pieces 6.41 ms against 5.92 ms for rows in x);
+  // rows walked in x for n <= 6 (128^3 n=4 515 against 532)
+  int piece = 1;
+  for (int r = std::min(20, my); r > 1 && piece == 1; r--)
+    if (my % r == 0) piece = r;
+  p->stream_walk = (n >= 7 && piece >= 4) ? 2 : 1;
   p->stream_rows = 0;
   p->items_bpc = 4;
   p->stream_planes = 0;
@@ -715,7 +752,8 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       p->lds_block = value;
       return FACES_OK;
     case FACES_OPT_STREAM_BPC:
-      if (value < 0 || value > 32) return fail(FACES_ERR_ARG, "blocks per CU %d outside [0,32]", value);
+      if (value < -4 || value == -1 || value > 32)
+        return fail(FACES_ERR_ARG, "blocks per CU %d outside [-4,-2] and [0,32]", value);
       p->stream_bpc = value;
       return FACES_OK;
     case FACES_OPT_STREAM_WALK:
@@ -774,7 +812,7 @@ int faces_plan_tune(faces_plan *p, double *u, int reps, void *stream, double *ou
     const int wr[5][2] = {{1, 1}, {2, 0}, {2, 10}, {3, 2}, {3, 4}};
     for (const auto &w : wr)
       for (int sl : segs)
-        for (int bpc : {1, 2, 3, 4}) cands.push_back({FACES_VARIANT_STREAM, w[0], w[1], sl, bpc});
+        for (int bpc : {1, 2, 3, 4, -2, -3}) cands.push_back({FACES_VARIANT_STREAM, w[0], w[1], sl, bpc});
   }
   p->seg_len = seg0;
   if (items_ok(p))

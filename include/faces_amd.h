@@ -122,8 +122,11 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *   FACES_OPT_SEG            elements per work item (0 = default)
  *   FACES_OPT_XCD            1 = contiguous work ranges per XCD
  *   FACES_OPT_LDS_BLOCK      threads per block of the LDS kernel
- *   FACES_OPT_STREAM_BPC     STREAM kernel: persistent blocks per CU (0 = one
- *                            work unit per block)
+ *   FACES_OPT_STREAM_BPC     STREAM kernel: persistent blocks per CU (1-32);
+ *                            0 = one work unit per block; -2, -3, -4 = one
+ *                            work unit per block, at most 2, 3, 4 blocks
+ *                            resident per CU (each block reserves that share
+ *                            of the CU's LDS)
  *   FACES_OPT_STREAM_WALK    STREAM kernel traversal: 0 = independent items,
  *                            1 = a block walks element rows in x carrying the
  *                            seam column (default for n <= 6), 2 = a block

@@ -100,7 +100,8 @@ def test_periodic_variants_bitexact(torch_cuda, shape, variant):
 
 @pytest.mark.parametrize("walk,rows", [(0, 1), (1, 1), (2, 1), (2, 2), (2, 3), (2, 64), (3, 1), (3, 2),
                                        (3, 3), (3, 4)])
-@pytest.mark.parametrize("seg,bpc,xcd", [(1, 0, 1), (2, 1, 0), (3, 2, 1), (4, 3, 1)])
+@pytest.mark.parametrize("seg,bpc,xcd", [(1, 0, 1), (2, 1, 0), (3, 2, 1), (4, 3, 1), (4, -2, 1), (2, -3, 0),
+                                         (3, -4, 1)])
 @pytest.mark.parametrize("shape", [(7, 12, 6, 3), (8, 4, 3, 5), (5, 6, 1, 2), (4, 3, 4, 1),
                                    (6, 2, 2, 2)])
 def test_stream_walks_bitexact(torch_cuda, walk, rows, seg, bpc, xcd, shape):
@@ -311,6 +312,9 @@ def test_invalid_arguments(torch_cuda):
         f.set_path(fa.PATH_FUSED)  # rank with a neighbour needs the staged path
     with pytest.raises(fa.FacesError):
         f.set_tuning(100, 0, 0)
+    for bad in (-1, -5, 33):
+        with pytest.raises(fa.FacesError):
+            f.set_option(fa.OPT_STREAM_BPC, bad)
     f.close()
 
 
