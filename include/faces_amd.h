@@ -163,8 +163,9 @@ int faces_plan_get_option(const faces_plan *plan, int option, int *value);
  * blocks per CU, on this plan's grid, and keeps the fastest.  Every trial
  * shares `u` 1 + reps times (compute = 1): the field changes as after that
  * many shares, so tune on a scratch copy.  Synchronises `stream`.  out (may
- * be NULL), 5 doubles: variant, walk, elements per item, blocks per CU, µs per
- * share of the choice. */
+ * be NULL), 5 doubles: variant, walk, elements per item, blocks per CU (as
+ * FACES_OPT_STREAM_BPC: negative = one work unit per block, at most that many
+ * resident per CU), µs per share of the choice. */
 int faces_plan_tune(faces_plan *plan, double *u, int reps, void *stream, double *out);
 
 /* Neighbour rank of message slot 0..25, or -1 for none (MPI_PROC_NULL) --
