@@ -299,6 +299,33 @@ def test_c2_configuration(torch_cuda):
     assert_same(got, want, shape)
 
 
+@pytest.mark.parametrize("shape,walk,bpc", [((8, 48, 48, 48), 2, -2), ((12, 30, 24, 30), 2, -2),
+                                            ((5, 60, 60, 60), 1, -3), ((8, 32, 32, 32), 2, 2),
+                                            ((9, 30, 46, 40), 1, -2)])
+def test_default_launch_large_fields(torch_cuda, shape, walk, bpc):
+    """The default traversal and block mode of fields of a few hundred MB
+    (one unit per block with a residency cap for n >= 8 from 256 MiB and for
+    n <= 5; rows in x where my has no divisor from 4 to 20), random values,
+    two shares, bit-exact."""
+    n, mx, my, mz = shape
+    print("case default launch", shape, flush=True)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    assert f.get_option(fa.OPT_ACTIVE_KERNEL) == fa.VARIANT_STREAM
+    assert (f.get_option(fa.OPT_STREAM_WALK), f.get_option(fa.OPT_STREAM_BPC)) == (walk, bpc)
+    d = torch_cuda.empty(fa.field_doubles(*shape), dtype=torch_cuda.float64, device="cuda")
+    fa.fill(d, *shape, kind=fa.FILL_RAND, seed=77)
+    for _ in range(2):
+        f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    del d
+    torch_cuda.cuda.empty_cache()
+    want = oracle.new_field(*shape, fa.FILL_RAND, seed=77)
+    for _ in range(2):
+        oracle.share_sweep(want, *shape, periodic=True)
+    assert_same(got, want, shape)
+
+
 def test_invalid_arguments(torch_cuda):
     with pytest.raises(fa.FacesError) as e:
         fa.Faces(0, 1, 1, 1, 2, 2, 2, 1)
