@@ -312,18 +312,23 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, bool skip, hipStream_t 
         if (p->my % r == 0) return r;
       return 1;
     };
+    const bool small = 8.0 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps) < 256.0 * 1024 * 1024;
     I.rows = divisor_upto(p->stream_rows > 0   ? p->stream_rows
                           : (I.walk == 3)      ? 4
                           : (p->stream_bpc < 0) ? 10
-                                                : 20);
+                          : small              ? 8
+                                               : 20);
     // Default column pieces of walk 2: up to 20 rows for persistent blocks
-    // (measured best on 60^3 n=7), up to 10 for one piece per block (60^3
+    // (measured best on 60^3 n=7), 8 on fields under 256 MiB (40^3 n=7: 69.9
+    // us against 74.4 for 10 rows), up to 10 for one piece per block (60^3
     // n=8 385 us against 396, 40^3 n=12 395 against 409), shorter while they
-    // would give fewer than 4 pieces per resident block (40^3 n=10, 2 blocks
-    // per CU: 10 rows 231 us, 20 rows 240 us).
+    // would give fewer than 4 pieces per block (48^3 n=7: 12 rows 135 us, 16
+    // rows 140; 40^3 n=10, 2 one-piece blocks per CU: 10 rows 231 us, 20 rows
+    // 240), 2 on fields under 256 MiB (32^3 n=8, six fields in turn: 8 rows
+    // 63.8 us, 4 rows 66.3).
     if (I.walk == 2 && p->stream_rows == 0 && p->stream_bpc != 0) {
-      const long nbk = (long)device_cus() * (p->stream_bpc > 0 ? p->stream_bpc : -p->stream_bpc);
-      while (I.rows > 1 && (long)p->mz * I.nseg * (p->my / I.rows) < 4 * nbk) I.rows = divisor_upto(I.rows - 1);
+      const long want = (p->stream_bpc > 0 ? (small ? 2L : 4L) * p->stream_bpc : 4L * -p->stream_bpc) * device_cus();
+      while (I.rows > 1 && (long)p->mz * I.nseg * (p->my / I.rows) < want) I.rows = divisor_upto(I.rows - 1);
     }
   }
   const long units = (I.walk == 1)   ? (long)p->my * p->mz
