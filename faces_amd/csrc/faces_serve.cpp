@@ -40,6 +40,11 @@ using Clock = std::chrono::steady_clock;
 // Requests at or above this footprint get a launch of their own.
 constexpr double kLargeBytes = 64.0 * 1024 * 1024;
 
+// Blocks per CU of a batched launch (each takes one contiguous chunk of the
+// launch's work list): 4 (2000 requests queued at once: 94k req/s, against
+// 79k with 32, the batched form's default, while two launches overlap).
+constexpr int kServeBlocksPerCu = 4;
+
 double now_us(Clock::time_point t0) { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
 
 struct Request {
@@ -127,7 +132,7 @@ int faces_server::launch(Slot &s, std::vector<Request *> &batch)
     D.ngrids = (int)g.size();
     D.nunits = nunits;
     D.total = total;
-    st = launch_batch(D, batch_blocks(nunits, 4), lds, s.stream);
+    st = launch_batch(D, batch_blocks(nunits, kServeBlocksPerCu), lds, s.stream);
     if (st) return st;
   }
   for (Request *r : other) {

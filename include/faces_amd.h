@@ -219,8 +219,9 @@ int faces_batch_share(faces_batch *batch, int compute, void *stream);
  * against). */
 int faces_batch_share_per_grid(faces_batch *batch, int compute, void *stream);
 
-/* Launch shape: threads per block, elements per work item (0 = whole row),
- * persistent blocks per CU (0 = one block per work item). */
+/* Launch shape: threads per block (0 or 256), longest work item in elements
+ * (1-64), blocks per CU (1-256; each block takes one contiguous chunk of the
+ * work list of equal estimated cost); 0 keeps the current value. */
 int faces_batch_set_tuning(faces_batch *batch, int block, int seg_len, int blocks_per_cu);
 
 /* Times every candidate launch shape on the bound fields (reps replays each,
