@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served leg")
     ap.add_argument("--grids", type=int, default=256, help="grids in the batched leg")
+    ap.add_argument("--batch-tuning", default=None,
+                    help="batched leg launch shape SEG,BLOCKS_PER_CU instead of the tuner's")
     ap.add_argument("--requests", type=int, default=2000, help="requests in the served leg")
     ap.add_argument("--streams", type=int, default=2, help="launch slots of the served leg")
     return ap.parse_args()
@@ -108,7 +110,7 @@ def event_ms(torch, fn, reps, stream):
     return s.elapsed_time(e) / reps
 
 
-def batched_leg(torch, fa, ngrids, steps, warmup):
+def batched_leg(torch, fa, ngrids, steps, warmup, tuning=None):
     from faces_amd.batch import Batch
     shapes = mixed_shapes(ngrids, seed=4)
     fields = [torch.empty(fa.field_doubles(*sh), dtype=torch.float64, device="cuda")
@@ -123,7 +125,12 @@ def batched_leg(torch, fa, ngrids, steps, warmup):
     b = Batch(shapes)
     b.set_fields(fields)
     stream = torch.cuda.current_stream()
-    tuned = b.tune(reps=5, stream=stream)
+    if tuning:  # fixed launch shape instead of the tuner: "SEG,BLOCKS_PER_CU"
+        seg, bpc = (int(x) for x in tuning.split(","))
+        b.set_tuning(256, seg, bpc)
+        tuned = {"seg_len": seg, "blocks_per_cu": bpc}
+    else:
+        tuned = b.tune(reps=5, stream=stream)
     refill()
     for _ in range(warmup):
         b.share(stream=stream)
@@ -336,7 +343,7 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(n, mx, my, mz, kind, args.cpu_seconds)
 
-    batched = None if args.no_batch else batched_leg(torch, fa, args.grids, 50, 5)
+    batched = None if args.no_batch else batched_leg(torch, fa, args.grids, 50, 5, args.batch_tuning)
     served = None if args.no_serve else served_leg(torch, fa, args.requests, args.streams)
 
     per_sorted = sorted(per)
