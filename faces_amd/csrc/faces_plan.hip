@@ -630,7 +630,10 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
     if (my % r == 0) piece = r;
   p->stream_walk = (n >= 7 && piece >= 4) ? 2 : 1;
   p->stream_rows = 0;
-  p->items_bpc = 4;
+  // work-list kernel on one grid (rows without a suitable divisor): 16
+  // blocks per CU for n >= 7 (61x60x60 n=7 320 us against 382 us for 4), 32
+  // for n <= 6 (79x80x80 n=6 475 against 565)
+  p->items_bpc = (n >= 7) ? 16 : 32;
   p->stream_planes = 0;
 
   // Message sizes, gpu/v-26-messages/Faces.cc:44-47, 64-66.
@@ -713,7 +716,7 @@ int faces_plan_set_variant(faces_plan *p, int variant, int block)
   if (variant < FACES_VARIANT_AUTO || variant > FACES_VARIANT_ITEMS)
     return fail(FACES_ERR_ARG, "unknown kernel variant %d", variant);
   if (variant == FACES_VARIANT_ITEMS) {
-    if (block < 0 || block > 4) return fail(FACES_ERR_ARG, "blocks per CU %d outside [0,4]", block);
+    if (block < 0 || block > 256) return fail(FACES_ERR_ARG, "blocks per CU %d outside [0,256]", block);
     if (!items_ok(p))
       return fail(FACES_ERR_ARG, "item kernel does not fit this grid (n=%d, mx=%d)", p->n, p->mx);
     p->variant = variant;
@@ -821,7 +824,7 @@ int faces_plan_tune(faces_plan *p, double *u, int reps, void *stream, double *ou
   }
   p->seg_len = seg0;
   if (items_ok(p))
-    for (int bpc : {2, 4}) cands.push_back({FACES_VARIANT_ITEMS, p->stream_walk, p->stream_rows, 0, bpc});
+    for (int bpc : {2, 4, 8, 16, 32, 64}) cands.push_back({FACES_VARIANT_ITEMS, p->stream_walk, p->stream_rows, 0, bpc});
   if (cands.empty()) return fail(FACES_ERR_STATE, "no tunable kernel for this grid");
 
   const int v0 = p->variant, w0 = p->stream_walk, r0 = p->stream_rows, b0 = p->stream_bpc, i0 = p->items_bpc;

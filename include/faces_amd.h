@@ -107,7 +107,9 @@ int faces_plan_set_tuning(faces_plan *plan, int block, int seg_len, int xcd_rema
  * a suitable divisor), else STREAM.
  * block: threads per block of the LDS kernel (64..512, multiple of 64),
  * persistent blocks per CU of the STREAM kernel (1..32, or -1 for one work
- * item per block) or of the ITEMS kernel (1..4); 0 keeps the current value. */
+ * item per block; FACES_OPT_STREAM_BPC also takes the capped forms) or
+ * blocks per CU of the ITEMS kernel (1..256, each block one contiguous chunk
+ * of the work list); 0 keeps the current value. */
 #define FACES_VARIANT_AUTO 0
 #define FACES_VARIANT_DIRECT 1
 #define FACES_VARIANT_LDS 2

@@ -91,7 +91,8 @@ def test_periodic_fused_bitexact(torch_cuda, shape, kind):
                                      (fa.VARIANT_LDS, 256), (fa.VARIANT_LDS, 512),
                                      (fa.VARIANT_STREAM, 1), (fa.VARIANT_STREAM, 8),
                                      (fa.VARIANT_STREAM, -1), (fa.VARIANT_ITEMS, 1),
-                                     (fa.VARIANT_ITEMS, 4)])
+                                     (fa.VARIANT_ITEMS, 4), (fa.VARIANT_ITEMS, 32),
+                                     (fa.VARIANT_ITEMS, 256)])
 def test_periodic_variants_bitexact(torch_cuda, shape, variant):
     print("case periodic variant", shape, variant, flush=True)
     u = oracle.new_field(*shape, fa.FILL_RAND, seed=202)
