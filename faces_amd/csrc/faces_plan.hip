@@ -176,7 +176,7 @@ SurfDesc surf_desc(const faces_plan *p, const Bound &b)
 
 int grid_for(long work, int block)
 {
-  const long cap = (long)device_cus() * 8;
+  const long cap = (long)device_cus() * 32;
   long g = (work + block - 1) / block;
   if (g > cap) g = cap;
   if (g < 1) g = 1;
