@@ -1,6 +1,7 @@
 // faces_plan.hip -- the plain exchange + DSS step behind the C ABI
 // (replaces class Faces, gpu/v-26-messages/Faces.{h,cc}).
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -32,35 +33,36 @@ int hip_fail(hipError_t e, const char *what)
   return fail(st, "%s: %s (%s)", what, hipGetErrorName(e), hipGetErrorString(e));
 }
 
+// Per-device attributes, cached (plans and servers may query them from
+// several threads).
 int device_cus()
 {
-  static int cached[64] = {0};
+  static std::atomic<int> cached[64] = {};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cached[dev] == 0) {
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0)
-      cus = 256;
-    cached[dev] = cus;
+  int v = cached[dev].load(std::memory_order_relaxed);
+  if (v == 0) {
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+      v = 256;
+    cached[dev].store(v, std::memory_order_relaxed);
   }
-  return cached[dev];
+  return v;
 }
 
 // LDS bytes per CU (160 KiB on gfx950).
 int device_lds_per_cu()
 {
-  static int cached[64] = {0};
+  static std::atomic<int> cached[64] = {};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 160 * 1024;
-  if (cached[dev] == 0) {
-    int b = 0;
-    if (hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
-        b <= 0)
-      b = 160 * 1024;
-    cached[dev] = b;
+  int v = cached[dev].load(std::memory_order_relaxed);
+  if (v == 0) {
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        v <= 0)
+      v = 160 * 1024;
+    cached[dev].store(v, std::memory_order_relaxed);
   }
-  return cached[dev];
+  return v;
 }
 
 }  // namespace faces
