@@ -66,7 +66,8 @@ def parse():
                     help="plan option NAME=VALUE (faces_amd OPT_* without the prefix)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-copy", action="store_true", help="skip the copy-kernel reference")
+    ap.add_argument("--no-copy", action="store_true",
+                    help="skip the copy and in-place stream references")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every share on its own instead of replaying a graph of --ni shares")
     ap.add_argument("--ni", type=int, default=10, help="shares per captured graph")
@@ -230,6 +231,17 @@ def copy_reference_gbs(torch, nbytes, reps=20):
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
+def inplace_stream_gbs(torch, fa, nbytes, reps=20):
+    """Read+write bandwidth of the library's plain in-place streaming kernel
+    over a buffer of the same size: the ceiling of an in-place step."""
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    gbs = fa.stream_ceiling(a, reps=reps, stream=torch.cuda.current_stream())
+    del a
+    return gbs
+
+
 def cpu_baseline(n, mx, my, mz, kind, budget_s):
     """The oracle's OpenMP sweep on the host cores: same workload, bounded."""
     import numpy as np  # noqa: F401
@@ -339,6 +351,7 @@ def main():
     torch.cuda.synchronize()
 
     copy_gbs = None if args.no_copy else copy_reference_gbs(torch, int(8 * total))
+    stream_gbs = None if args.no_copy else inplace_stream_gbs(torch, fa, int(8 * total))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n, mx, my, mz, kind, args.cpu_seconds)
@@ -376,6 +389,8 @@ def main():
                      "traffic": load_traffic(args.config)},
         "copy_reference_gbs": None if copy_gbs is None else round(copy_gbs, 1),
         "frac_of_copy": None if copy_gbs is None else round(gbs / copy_gbs, 4),
+        "inplace_stream_gbs": None if stream_gbs is None else round(stream_gbs, 1),
+        "frac_of_inplace_stream": None if stream_gbs is None else round(gbs / stream_gbs, 4),
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,

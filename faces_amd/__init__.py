@@ -19,7 +19,7 @@ import os
 import re
 
 __all__ = [
-    "FacesError", "Faces", "Batch", "Server", "fill", "field_doubles", "lib",
+    "FacesError", "Faces", "Batch", "Server", "fill", "field_doubles", "stream_ceiling", "lib",
     "LIB_PATH", "OPEN", "PERIODIC", "PATH_AUTO", "PATH_FUSED", "PATH_STAGED",
     "PATH_STAGED_DIRECT", "PATH_SIX", "FILL_INT", "FILL_RAND", "FILL_REF", "header_symbols",
 ]
@@ -92,6 +92,7 @@ _sig("faces_inner", _c_int, _vp, _vp, _vp)
 _sig("faces_unpack", _c_int, _vp, _vp, _vp)
 _sig("faces_fill", _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _c_long, _c_int, _c_u64, _vp)
 _sig("faces_field_doubles", _c_long, _c_int, _c_int, _c_int, _c_int, _c_long)
+_sig("faces_stream_ceiling", _c_int, _vp, _c_long, _c_int, _vp, ctypes.POINTER(ctypes.c_double))
 
 
 def lib() -> ctypes.CDLL:
@@ -131,6 +132,17 @@ def _stream(stream) -> int | None:
 
 def field_doubles(n, mx, my, mz, plane_stride=0) -> int:
     return int(_lib.faces_field_doubles(n, mx, my, mz, plane_stride))
+
+
+def stream_ceiling(u, n=None, reps=20, stream=None) -> float:
+    """In-place read+write GB/s of a plain streaming kernel over the first n
+    doubles of u (all of a tensor by default); values are unchanged."""
+    if n is None:
+        n = u.numel()
+    out = ctypes.c_double()
+    _check(_lib.faces_stream_ceiling(_addr(u), n, reps, _stream(stream), ctypes.byref(out)),
+           "faces_stream_ceiling")
+    return out.value
 
 
 def fill(u, n, mx, my, mz, kind=FILL_INT, seed=42, plane_stride=0, stream=None) -> None:

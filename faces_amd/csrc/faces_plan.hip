@@ -2,6 +2,7 @@
 // (replaces class Faces, gpu/v-26-messages/Faces.{h,cc}).
 #include <algorithm>
 #include <atomic>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -982,6 +983,56 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
     if (st) return st;
   }
   return launch_surface(p, u, bound_recv(p->mask), s);
+}
+
+// In-place read-modify-write stream, one double2 per lane, one launch over
+// the buffer: the bandwidth ceiling of an in-place pass on this device.
+// `scale` is 1.0 at run time, so values stay as they are.
+static __global__ __launch_bounds__(256) void k_rmw_stream(double *__restrict__ u, long n, double scale)
+{
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  const long n2 = n / 2;
+  if (i < n2) {
+    double2 v = reinterpret_cast<double2 *>(u)[i];
+    v.x *= scale;
+    v.y *= scale;
+    reinterpret_cast<double2 *>(u)[i] = v;
+  } else if (i == n2 && (n & 1)) {
+    u[n - 1] *= scale;
+  }
+}
+
+int faces_stream_ceiling(double *u, long n, int reps, void *stream, double *gbs)
+{
+  if (!u || !gbs) return fail(FACES_ERR_ARG, "null argument");
+  if (reinterpret_cast<uintptr_t>(u) % 16) return fail(FACES_ERR_ARG, "buffer not 16-byte aligned");
+  if (n < 2) return fail(FACES_ERR_ARG, "buffer of %ld doubles", n);
+  if (reps < 1 || reps > 1000) return fail(FACES_ERR_ARG, "reps %d outside [1,1000]", reps);
+  const long blocks = (n / 2 + 1 + 255) / 256;
+  if (blocks > 0x7fffffffL) return fail(FACES_ERR_ARG, "buffer too large");
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return fail(FACES_ERR_HIP, "hipEventCreate");
+  }
+  int rc = FACES_OK;
+  float ms = 0.f;
+  for (int r = 0; r < 2 + reps && !rc; r++) {
+    if (r == 2 && hipEventRecord(e0, st) != hipSuccess) rc = fail(FACES_ERR_HIP, "hipEventRecord");
+    if (!rc) {
+      k_rmw_stream<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(u, n, 1.0);
+      if (hipGetLastError() != hipSuccess) rc = fail(FACES_ERR_HIP, "stream kernel launch");
+    }
+  }
+  if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = fail(FACES_ERR_HIP, "timing events");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  *gbs = ms > 0.f ? 16.0 * (double)n * reps / (ms * 1e-3) / 1e9 : 0.0;
+  return FACES_OK;
 }
 
 int faces_fill(double *u, int n, int mx, int my, int mz, long plane_stride, int kind, uint64_t seed,

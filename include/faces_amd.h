@@ -198,6 +198,14 @@ int faces_fill(double *u, int n, int mx, int my, int mz, long plane_stride, int 
 /* Doubles spanned by a field (including plane padding). */
 long faces_field_doubles(int n, int mx, int my, int mz, long plane_stride);
 
+/* Measurement helper, no counterpart in the reference: the in-place
+ * read-plus-write bandwidth (GB/s, 16 B per double per pass) of a plain
+ * streaming kernel over n doubles at u (16-byte aligned), timed over reps
+ * passes after two untimed ones -- the ceiling an in-place step can reach on
+ * this device.  Values are unchanged (each is multiplied by 1.0; a
+ * signalling NaN becomes quiet).  Synchronises `stream`. */
+int faces_stream_ceiling(double *u, long n, int reps, void *stream, double *gbs);
+
 /* ---- batched step: many independent grids, one launch -------------------- */
 /* No counterpart in the reference, which steps one grid per process
  * (gpu/main.cc:170-186). */

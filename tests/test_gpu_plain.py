@@ -327,6 +327,25 @@ def test_default_launch_large_fields(torch_cuda, shape, walk, bpc):
     assert_same(got, want, shape)
 
 
+@pytest.mark.parametrize("count", [2, 3, 1001, 2 ** 20 + 7])
+def test_stream_ceiling_leaves_values(torch_cuda, count):
+    """The in-place streaming probe reports a bandwidth and leaves every bit
+    of the buffer as it was (odd lengths included)."""
+    print("case stream ceiling", count, flush=True)
+    d = torch_cuda.empty(count, dtype=torch_cuda.float64, device="cuda")
+    d.uniform_(-1.0, 1.0)
+    d[0] = -0.0
+    before = to_host(torch_cuda, d)
+    gbs = fa.stream_ceiling(d, reps=3)
+    assert gbs > 0
+    assert np.array_equal(to_host(torch_cuda, d).view(np.uint64), before.view(np.uint64))
+    with pytest.raises(fa.FacesError) as e:
+        fa.stream_ceiling(int(d.data_ptr()) + 8, n=count - 1, reps=1)  # not 16-byte aligned
+    assert e.value.status == fa.ERR_ARG
+    with pytest.raises(fa.FacesError):
+        fa.stream_ceiling(d, n=1)
+
+
 def test_invalid_arguments(torch_cuda):
     with pytest.raises(fa.FacesError) as e:
         fa.Faces(0, 1, 1, 1, 2, 2, 2, 1)
