@@ -295,7 +295,7 @@ bool stream_ok(const faces_plan *p)
 }
 
 template <int N, int S>
-int launch_stream_ns(const faces_plan *p, const Geom &G, bool skip, hipStream_t st)
+int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t st)
 {
   StreamItems I;
   I.my = p->my;
@@ -307,7 +307,7 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, bool skip, hipStream_t 
                       : (unsigned)(8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps));
   I.xcd_remap = p->xcd_remap;
   I.walk = p->stream_walk;
-  I.skip_surface = skip ? 1 : 0;
+  I.skip_surface = mode;
   I.rows = 1;
   if (I.walk == 2 || I.walk == 3) {
     auto divisor_upto = [&](int want) {
@@ -351,59 +351,63 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, bool skip, hipStream_t 
     const size_t share = (size_t)device_lds_per_cu() / (size_t)(1 - p->stream_bpc) + 1024;
     lds_launch = std::max(lds, std::min(share, (size_t)64 * 1024));
   }
-  if (big)
+  if (mode == 2) {
+    if (big) return fail(FACES_ERR_ARG, "open-rank streaming kernel takes fields under 4 GB");
+    k_stream<N, S, false, true><<<dim3((unsigned)nb), dim3(kStreamThreads), lds_launch, st>>>(G, I);
+  } else if (big) {
     k_stream<N, S, true><<<dim3((unsigned)nb), dim3(kStreamThreads), lds_launch, st>>>(G, I);
-  else
+  } else {
     k_stream<N, S, false><<<dim3((unsigned)nb), dim3(kStreamThreads), lds_launch, st>>>(G, I);
+  }
   FACES_HIP(hipGetLastError());
   return FACES_OK;
 }
 
 template <int N>
-int launch_stream_n(const faces_plan *p, const Geom &G, int S, bool skip, hipStream_t st)
+int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, hipStream_t st)
 {
   switch (S) {
-    case 1: return launch_stream_ns<N, 1>(p, G, skip, st);
-    case 2: return launch_stream_ns<N, 2>(p, G, skip, st);
-    case 3: return launch_stream_ns<N, 3>(p, G, skip, st);
-    case 4: return launch_stream_ns<N, 4>(p, G, skip, st);
+    case 1: return launch_stream_ns<N, 1>(p, G, mode, st);
+    case 2: return launch_stream_ns<N, 2>(p, G, mode, st);
+    case 3: return launch_stream_ns<N, 3>(p, G, mode, st);
+    case 4: return launch_stream_ns<N, 4>(p, G, mode, st);
     default: break;
   }
   if constexpr (N >= 5 && N <= 8) {
-    if (S == 5) return launch_stream_ns<N, 5>(p, G, skip, st);
+    if (S == 5) return launch_stream_ns<N, 5>(p, G, mode, st);
   }
   if constexpr (N == 5) {
-    if (S == 10) return launch_stream_ns<N, 10>(p, G, skip, st);
+    if (S == 10) return launch_stream_ns<N, 10>(p, G, mode, st);
   }
   if constexpr (N <= 6) {
-    if (S == 6) return launch_stream_ns<N, 6>(p, G, skip, st);
-    if (S == 8) return launch_stream_ns<N, 8>(p, G, skip, st);
+    if (S == 6) return launch_stream_ns<N, 6>(p, G, mode, st);
+    if (S == 8) return launch_stream_ns<N, 8>(p, G, mode, st);
   }
   if constexpr (N <= 4) {
-    if (S == 16) return launch_stream_ns<N, 16>(p, G, skip, st);
+    if (S == 16) return launch_stream_ns<N, 16>(p, G, mode, st);
   }
   if constexpr (N == 2) {
-    if (S == 32) return launch_stream_ns<N, 32>(p, G, skip, st);
+    if (S == 32) return launch_stream_ns<N, 32>(p, G, mode, st);
   }
   return fail(FACES_ERR_ARG, "streaming kernel has no n=%d S=%d specialisation", N, S);
 }
 
-int launch_stream(const faces_plan *p, double *u, bool skip, hipStream_t st)
+int launch_stream(const faces_plan *p, double *u, int mode, hipStream_t st)
 {
   const Geom G = make_geom(p, u);
   const int S = stream_seg(p);
   switch (p->n) {
-    case 2: return launch_stream_n<2>(p, G, S, skip, st);
-    case 3: return launch_stream_n<3>(p, G, S, skip, st);
-    case 4: return launch_stream_n<4>(p, G, S, skip, st);
-    case 5: return launch_stream_n<5>(p, G, S, skip, st);
-    case 6: return launch_stream_n<6>(p, G, S, skip, st);
-    case 7: return launch_stream_n<7>(p, G, S, skip, st);
-    case 8: return launch_stream_n<8>(p, G, S, skip, st);
-    case 9: return launch_stream_n<9>(p, G, S, skip, st);
-    case 10: return launch_stream_n<10>(p, G, S, skip, st);
-    case 11: return launch_stream_n<11>(p, G, S, skip, st);
-    case 12: return launch_stream_n<12>(p, G, S, skip, st);
+    case 2: return launch_stream_n<2>(p, G, S, mode, st);
+    case 3: return launch_stream_n<3>(p, G, S, mode, st);
+    case 4: return launch_stream_n<4>(p, G, S, mode, st);
+    case 5: return launch_stream_n<5>(p, G, S, mode, st);
+    case 6: return launch_stream_n<6>(p, G, S, mode, st);
+    case 7: return launch_stream_n<7>(p, G, S, mode, st);
+    case 8: return launch_stream_n<8>(p, G, S, mode, st);
+    case 9: return launch_stream_n<9>(p, G, S, mode, st);
+    case 10: return launch_stream_n<10>(p, G, S, mode, st);
+    case 11: return launch_stream_n<11>(p, G, S, mode, st);
+    case 12: return launch_stream_n<12>(p, G, S, mode, st);
     default: return fail(FACES_ERR_ARG, "streaming kernel has no n=%d specialisation", p->n);
   }
 }
@@ -442,7 +446,7 @@ int launch_box(const faces_plan *p, double *u, bool skip_surface, hipStream_t st
   // nodes; the other kernels fall back to the direct one
   if (skip_surface && kern != FACES_VARIANT_STREAM)
     kern = (p->variant == FACES_VARIANT_AUTO && stream_ok(p)) ? FACES_VARIANT_STREAM : FACES_VARIANT_DIRECT;
-  if (kern == FACES_VARIANT_STREAM) return launch_stream(p, u, skip_surface, st);
+  if (kern == FACES_VARIANT_STREAM) return launch_stream(p, u, skip_surface ? 1 : 0, st);
   if (kern == FACES_VARIANT_ITEMS)
     return launch_single(u, p->n, p->mx, p->my, p->mz, p->ps, p->seg_len > 0 ? p->seg_len : 64,
                          p->items_bpc, st);
@@ -961,7 +965,11 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
       if (compute) return launch_box(p, u, false, s);
       return launch_surface(p, u, bound_wrap(), s);
     }
-    // Open rank without neighbours: interior nodes, then the rank surface.
+    // Open rank without neighbours: one streaming launch that sums each
+    // rank-surface node over its own side's copies; else interior nodes,
+    // then the rank surface.
+    if (compute && p->mask == 0 && active_kernel(p) == FACES_VARIANT_STREAM && !stream_big(p))
+      return launch_stream(p, u, 2, s);
     if (compute) {
       st = launch_box(p, u, true, s);
       if (st) return st;

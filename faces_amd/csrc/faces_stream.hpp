@@ -63,7 +63,10 @@ struct StreamItems {
                         // 3 groups of `rows` rows, segment by segment (seam and y rows carried)
   int rows;             // walk 3: rows per group; walk 2: rows per column piece (divides my)
   int skip_surface;     // 1: sum only nodes off the rank surface (open ranks:
-                        // the surface is summed with the received copies)
+                        // the surface is summed with the received copies);
+                        // 2: open rank without neighbours: a rank-surface
+                        // node sums only its copies on its own side of the
+                        // rank surface (the wrap partners form nodes of their own)
 };
 
 // LDS doubles of the streaming kernel: the item footprint plus, for the walks
@@ -80,7 +83,10 @@ __host__ __device__ constexpr int stream_lds_doubles(int walk, int rows)
 // with 32-bit offsets inside the plane; slots of the partner region that may
 // lie in either plane are loaded and stored once through each descriptor,
 // the lanes of the other plane sent out of range.
-template <int N, int S, bool BIG = false>
+// OPEN: an open rank without neighbours (skip_surface = 2): the copies on
+// each side of the rank surface are summed apart (instantiated apart, so the
+// periodic kernels keep their register allocation).
+template <int N, int S, bool BIG = false, bool OPEN = false>
 __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I)
 {
   using SH = StreamShape<N, S>;
@@ -403,14 +409,18 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     for (int i = 0; i < NN; i++) {
       const unsigned f = nflag[i];
       if (!(f & 8u)) continue;
-      // rank-surface nodes (global coordinate 0 on an axis: the periodic
-      // wrap pairs) are left to the surface kernel when skipping
-      if (skip_seam && ((f & skip_seam & 6u) || ((f & 17u) == 17u && (skip_seam & 1u)))) continue;
-      const bool cx = f & 1u, cy = f & 2u, cz = f & 4u;
+      // Axes on which this node's pair is a periodic wrap pair (global
+      // coordinate 0: the rank surface).  Skipping, such nodes are left to the
+      // surface kernel; open without neighbours, the copies on each side of
+      // those axes are summed apart (x pairs, then y, then z, over the other
+      // shared axes), each side being a node of its own.
+      const unsigned wrap = skip_seam & ((f & 6u) | ((f & 17u) == 17u ? 1u : 0u));
+      if (wrap) continue;  // rank-surface node: the surface kernel's, or the pass below
       const unsigned p0 = np[i][0] & 0xffff, p1 = np[i][0] >> 16;
       const unsigned p2 = np[i][1] & 0xffff, p3 = np[i][1] >> 16;
       const unsigned p4 = np[i][2] & 0xffff, p5 = np[i][2] >> 16;
       const unsigned p6 = np[i][3] & 0xffff, p7 = np[i][3] >> 16;
+      const bool cx = f & 1u, cy = f & 2u, cz = f & 4u;
       const double a0 = cx ? lds[p0] + lds[p1] : lds[p0];
       double q0 = a0;
       if (cy) q0 = a0 + (cx ? lds[p2] + lds[p3] : lds[p2]);
@@ -433,6 +443,68 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
         if (cy) {
           lds[p6] = res;
           if (cx) lds[p7] = res;
+        }
+      }
+    }
+    // Open rank without neighbours: the rank-surface nodes of this item
+    // (block-uniform: only items on the surface have any).  The copies on
+    // each side of the surface are summed apart, x pairs, then y, then z over
+    // the node's other shared axes; each side is a node of its own.
+    if constexpr (OPEN) {
+      if (skip_seam) {
+#pragma unroll
+        for (int i = 0; i < NN; i++) {
+          const unsigned f = nflag[i];
+          if (!(f & 8u)) continue;
+          const unsigned wrap = skip_seam & ((f & 6u) | ((f & 17u) == 17u ? 1u : 0u));
+          if (!wrap) continue;
+          // copy (sx | sy << 1 | sz << 2), side 1 = the partner across that
+          // axis; all eight are read at once (positions of copies the node
+          // does not have lie in the footprint too, and go unused)
+          const unsigned pp[8] = {np[i][0] & 0xffff, np[i][0] >> 16, np[i][1] & 0xffff, np[i][1] >> 16,
+                                  np[i][2] & 0xffff, np[i][2] >> 16, np[i][3] & 0xffff, np[i][3] >> 16};
+          double v[8];
+#pragma unroll
+          for (int m = 0; m < 8; m++) v[m] = lds[pp[m]];
+          const unsigned sh = (f & 7u) & ~wrap;  // axes summed within a side
+          // the tree level by level, for both sides of every axis: x pairs
+          // (or the side's copy), then y, then z
+          double A[2][2][2], Q[2][2][2], R[2][2][2];  // [gx][..][..]
+#pragma unroll
+          for (int gx = 0; gx < 2; gx++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              const double x0 = v[2 * k], x1 = v[2 * k + 1];
+              A[gx][k & 1][k >> 1] = (sh & 1u) ? x0 + x1 : (gx ? x1 : x0);
+            }
+#pragma unroll
+          for (int gx = 0; gx < 2; gx++)
+#pragma unroll
+            for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+              for (int az = 0; az < 2; az++)
+                Q[gx][gy][az] = (sh & 2u) ? A[gx][0][az] + A[gx][1][az] : A[gx][gy][az];
+#pragma unroll
+          for (int gx = 0; gx < 2; gx++)
+#pragma unroll
+            for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+              for (int gz = 0; gz < 2; gz++)
+                R[gx][gy][gz] = (sh & 4u) ? Q[gx][gy][0] + Q[gx][gy][1] : Q[gx][gy][gz];
+#pragma unroll
+          for (int m = 0; m < 8; m++) {
+            const bool ex = (!(m & 1) || (f & 1u)) && (!(m & 2) || (f & 2u)) && (!(m & 4) || (f & 4u));
+            if (!ex) continue;
+            // copy m's side on each split axis, picked with selects
+            const bool gx = (wrap & 1u) && (m & 1), gy = (wrap & 2u) && (m & 2), gz = (wrap & 4u) && (m & 4);
+            double rz[2][2];
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+              for (int y = 0; y < 2; y++) rz[x][y] = gz ? R[x][y][1] : R[x][y][0];
+            const double ry0 = gy ? rz[0][1] : rz[0][0], ry1 = gy ? rz[1][1] : rz[1][0];
+            lds[pp[m]] = gx ? ry1 : ry0;
+          }
         }
       }
     }

@@ -141,6 +141,33 @@ def test_open_bitexact(torch_cuda, shape):
     assert_same(got, run_oracle(u, shape, periodic=False), shape)
 
 
+@pytest.mark.parametrize("walk,rows,bpc", [(0, 1, 2), (1, 1, 2), (2, 2, 2), (2, 0, -2), (3, 2, 2),
+                                           (1, 1, -3)])
+@pytest.mark.parametrize("shape,seg", [((7, 12, 6, 4), 4), ((7, 1, 1, 1), 1), ((5, 10, 3, 2), 5),
+                                       ((4, 32, 2, 3), 16), ((8, 3, 4, 5), 3), ((12, 2, 2, 1), 1),
+                                       ((2, 4, 1, 3), 4), ((6, 8, 5, 1), 8)])
+def test_open_rank_streaming_bitexact(torch_cuda, walk, rows, bpc, shape, seg):
+    """Open boundary on a 1x1x1 rank grid (the reference's own mode) in one
+    streaming launch: each rank-surface node sums only the copies on its side
+    of the surface; equal to the open sweep for every traversal."""
+    n, mx, my, mz = shape
+    print("case open streaming", shape, seg, walk, rows, bpc, flush=True)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=808)
+    d = to_dev(torch_cuda, u)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.OPEN)
+    f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+    f.set_option(fa.OPT_SEG, seg)
+    f.set_option(fa.OPT_STREAM_WALK, walk)
+    f.set_option(fa.OPT_STREAM_ROWS, rows)
+    f.set_option(fa.OPT_STREAM_BPC, bpc)
+    assert f.get_option(fa.OPT_ACTIVE_KERNEL) == fa.VARIANT_STREAM
+    for _ in range(2):
+        f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    assert_same(got, run_oracle(u, shape, periodic=False, nshare=2), shape)
+
+
 @pytest.mark.parametrize("shape", [(5, 4, 4, 4), (7, 3, 3, 2), (2, 2, 3, 1), (12, 2, 1, 2)])
 @pytest.mark.parametrize("periodic", [True, False])
 def test_compute_false(torch_cuda, shape, periodic):

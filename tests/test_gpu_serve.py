@@ -178,6 +178,39 @@ def test_ticket_waited_once(torch_cuda):
     srv.close()
 
 
+def test_submit_from_several_threads(torch_cuda):
+    """Four host threads submit and wait for their own requests at once; every
+    result is bit-exact and the server's counts add up."""
+    import threading
+    print("case serve threads", flush=True)
+    shapes = mix_shapes(48, seed=23)
+    fields = [field(torch_cuda, s, 800 + i) for i, s in enumerate(shapes)]
+    torch_cuda.cuda.synchronize()
+    srv = Server(footprint_cap=8e9, max_batch=16, nstreams=2)
+    errors = []
+
+    def worker(k):
+        try:
+            ts = [srv.submit(fields[i], *shapes[i]) for i in range(k, len(shapes), 4)]
+            for t in ts:
+                srv.wait(t, timeout_s=60.0)
+        except Exception as x:  # reported below, in the main thread
+            errors.append(repr(x))
+
+    th = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errors, errors
+    st = srv.stats()
+    assert st["admitted"] == len(shapes) and st["completed"] == len(shapes)
+    for i, s in enumerate(shapes):
+        r = oracle.compare(fields[i].cpu().numpy(), oracle_one(s, 800 + i), *s)
+        assert r["ndiff"] == 0, (s, r)
+    srv.close()
+
+
 def test_large_requests_launch_alone(torch_cuda):
     """A request of 64 MiB or more gets a launch of its own; the small ones
     around it are stepped in other launches, all bit-exact."""
