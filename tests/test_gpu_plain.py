@@ -548,11 +548,13 @@ def test_field_above_4gb_bitexact(torch_cuda):
 
 
 @pytest.mark.parametrize("shape", [(7, 12, 6, 4), (5, 8, 4, 3), (8, 7, 3, 2), (4, 16, 4, 2)])
-def test_plan_tune_then_bitexact(torch_cuda, shape):
-    """The autotuner picks a valid launch shape; a share with it is exact."""
+@pytest.mark.parametrize("boundary", [fa.PERIODIC, fa.OPEN])
+def test_plan_tune_then_bitexact(torch_cuda, shape, boundary):
+    """The autotuner picks a valid launch shape; a share with it is exact
+    (periodic, and open on one rank: the one-launch open mode)."""
     n, mx, my, mz = shape
-    print("case plan tune", shape, flush=True)
-    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    print("case plan tune", shape, boundary, flush=True)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=boundary)
     scratch = torch_cuda.empty(fa.field_doubles(*shape), dtype=torch_cuda.float64, device="cuda")
     fa.fill(scratch, *shape, kind=fa.FILL_RAND, seed=9)
     res = f.tune(scratch, reps=2)
@@ -563,4 +565,4 @@ def test_plan_tune_then_bitexact(torch_cuda, shape):
     f.share(d)
     got = to_host(torch_cuda, d)
     f.close()
-    assert_same(got, run_oracle(u, shape), shape)
+    assert_same(got, run_oracle(u, shape, periodic=boundary == fa.PERIODIC), shape)
