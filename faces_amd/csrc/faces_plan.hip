@@ -308,6 +308,7 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t s
   I.xcd_remap = p->xcd_remap;
   I.walk = p->stream_walk;
   I.skip_surface = mode;
+  I.nbr_mask = p->mask;
   I.rows = 1;
   if (I.walk == 2 || I.walk == 3) {
     auto divisor_upto = [&](int want) {
@@ -699,8 +700,6 @@ int faces_plan_set_path(faces_plan *p, int path)
   if (!p) return fail(FACES_ERR_ARG, "null plan");
   if (path < FACES_PATH_AUTO || path > FACES_PATH_SIX)
     return fail(FACES_ERR_ARG, "unknown path %d", path);
-  if (path == FACES_PATH_FUSED && p->boundary == FACES_BOUNDARY_OPEN && p->mask != 0)
-    return fail(FACES_ERR_ARG, "fused path needs a periodic grid or a rank without neighbours");
   p->path = path;
   return FACES_OK;
 }
@@ -940,8 +939,12 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
   if (st) return st;
   hipStream_t s = (hipStream_t)stream;
   int path = p->path;
+  // An open rank takes the fused path where one streaming launch sums it
+  // (its neighbours stood in for by self halos), else the staged one.
+  const bool open_stream = p->boundary == FACES_BOUNDARY_OPEN && compute &&
+                           active_kernel(p) == FACES_VARIANT_STREAM && !stream_big(p);
   if (path == FACES_PATH_AUTO)
-    path = (p->boundary == FACES_BOUNDARY_PERIODIC || p->mask == 0 || p->mask == 63)
+    path = (p->boundary == FACES_BOUNDARY_PERIODIC || p->mask == 0 || p->mask == 63 || open_stream)
                ? FACES_PATH_FUSED
                : FACES_PATH_STAGED_DIRECT;
   if (path == FACES_PATH_SIX) {
@@ -965,11 +968,14 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
       if (compute) return launch_box(p, u, false, s);
       return launch_surface(p, u, bound_wrap(), s);
     }
-    // Open rank without neighbours: one streaming launch that sums each
-    // rank-surface node over its own side's copies; else interior nodes,
-    // then the rank surface.
-    if (compute && p->mask == 0 && active_kernel(p) == FACES_VARIANT_STREAM && !stream_big(p))
-      return launch_stream(p, u, 2, s);
+    // Other open ranks: one streaming launch in which each rank-surface node
+    // sums the copies on its side, and those across the surface where the
+    // rank has a neighbour on that side; without the streaming kernel, a
+    // rank without neighbours sums interior nodes, then the rank surface.
+    if (open_stream) return launch_stream(p, u, 2, s);
+    if (p->mask != 0)
+      return fail(FACES_ERR_ARG, "fused path of a rank with neighbours needs the streaming kernel "
+                                 "(fields under 4 GB, compute = 1): use the staged path");
     if (compute) {
       st = launch_box(p, u, true, s);
       if (st) return st;

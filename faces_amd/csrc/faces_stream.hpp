@@ -67,6 +67,9 @@ struct StreamItems {
                         // 2: open rank without neighbours: a rank-surface
                         // node sums only its copies on its own side of the
                         // rank surface (the wrap partners form nodes of their own)
+  int nbr_mask;         // skip_surface = 2: face neighbours of the rank, bit 2a (2a+1) for
+                        // the low (high) side of axis a: that side's surface nodes also
+                        // sum the copies across the surface (stood in for by self halos)
 };
 
 // LDS doubles of the streaming kernel: the item footprint plus, for the walks
@@ -83,9 +86,10 @@ __host__ __device__ constexpr int stream_lds_doubles(int walk, int rows)
 // with 32-bit offsets inside the plane; slots of the partner region that may
 // lie in either plane are loaded and stored once through each descriptor,
 // the lanes of the other plane sent out of range.
-// OPEN: an open rank without neighbours (skip_surface = 2): the copies on
-// each side of the rank surface are summed apart (instantiated apart, so the
-// periodic kernels keep their register allocation).
+// OPEN: an open rank (skip_surface = 2): a rank-surface node sums the copies
+// on its own side of the surface, and those across it when the rank has a
+// neighbour on that side, stood in for by a self halo (instantiated apart,
+// so the periodic kernels keep their register allocation).
 template <int N, int S, bool BIG = false, bool OPEN = false>
 __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I)
 {
@@ -446,10 +450,10 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
         }
       }
     }
-    // Open rank without neighbours: the rank-surface nodes of this item
-    // (block-uniform: only items on the surface have any).  The copies on
-    // each side of the surface are summed apart, x pairs, then y, then z over
-    // the node's other shared axes; each side is a node of its own.
+    // Open rank: the rank-surface nodes of this item (block-uniform: only
+    // items on the surface have any).  Each side of the surface is a node of
+    // its own: it sums its copies, and those across the surface when the rank
+    // has a neighbour on that side, x pairs, then y, then z.
     if constexpr (OPEN) {
       if (skip_seam) {
 #pragma unroll
@@ -466,7 +470,15 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
           double v[8];
 #pragma unroll
           for (int m = 0; m < 8; m++) v[m] = lds[pp[m]];
-          const unsigned sh = (f & 7u) & ~wrap;  // axes summed within a side
+          // pair[a][side]: that side of axis a sums both copies -- a pair
+          // inside the rank, or across the surface with a neighbour there
+          const unsigned nb = (unsigned)I.nbr_mask;
+          bool pair[3][2];
+#pragma unroll
+          for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int sd = 0; sd < 2; sd++)
+              pair[a][sd] = (f & (1u << a)) && (!(wrap & (1u << a)) || (nb & (1u << (2 * a + sd))));
           // the tree level by level, for both sides of every axis: x pairs
           // (or the side's copy), then y, then z
           double A[2][2][2], Q[2][2][2], R[2][2][2];  // [gx][..][..]
@@ -475,7 +487,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
 #pragma unroll
             for (int k = 0; k < 4; k++) {
               const double x0 = v[2 * k], x1 = v[2 * k + 1];
-              A[gx][k & 1][k >> 1] = (sh & 1u) ? x0 + x1 : (gx ? x1 : x0);
+              A[gx][k & 1][k >> 1] = pair[0][gx] ? x0 + x1 : (gx ? x1 : x0);
             }
 #pragma unroll
           for (int gx = 0; gx < 2; gx++)
@@ -483,14 +495,14 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
             for (int gy = 0; gy < 2; gy++)
 #pragma unroll
               for (int az = 0; az < 2; az++)
-                Q[gx][gy][az] = (sh & 2u) ? A[gx][0][az] + A[gx][1][az] : A[gx][gy][az];
+                Q[gx][gy][az] = pair[1][gy] ? A[gx][0][az] + A[gx][1][az] : A[gx][gy][az];
 #pragma unroll
           for (int gx = 0; gx < 2; gx++)
 #pragma unroll
             for (int gy = 0; gy < 2; gy++)
 #pragma unroll
               for (int gz = 0; gz < 2; gz++)
-                R[gx][gy][gz] = (sh & 4u) ? Q[gx][gy][0] + Q[gx][gy][1] : Q[gx][gy][gz];
+                R[gx][gy][gz] = pair[2][gz] ? Q[gx][gy][0] + Q[gx][gy][1] : Q[gx][gy][gz];
 #pragma unroll
           for (int m = 0; m < 8; m++) {
             const bool ex = (!(m & 1) || (f & 1u)) && (!(m & 2) || (f & 2u)) && (!(m & 4) || (f & 4u));

@@ -47,7 +47,10 @@ extern "C" {
 
 /* Execution paths of faces_share. */
 #define FACES_PATH_AUTO 0          /* fused when possible, else staged */
-#define FACES_PATH_FUSED 1         /* gather-sum-scatter in place, one launch */
+#define FACES_PATH_FUSED 1         /* gather-sum-scatter in place, one launch; for
+                                      an open rank with neighbours (self halos)
+                                      it needs the streaming kernel, compute = 1
+                                      and a field under 4 GB */
 #define FACES_PATH_STAGED 2        /* pack -> exchange copy -> inner + unpack */
 #define FACES_PATH_STAGED_DIRECT 3 /* pack straight into receive slots */
 #define FACES_PATH_SIX 4           /* the 6-message schedule: z, y, x face phases

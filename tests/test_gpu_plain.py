@@ -258,10 +258,38 @@ def test_rank_positions_open(torch_cuda, lgrid):
         u = oracle.new_field(*shape, fa.FILL_RAND, seed=rank + 1)
         want = u.copy()
         oracle.share_staged(want, *shape, nbr_mask=mask)
-        for path in (fa.PATH_AUTO, fa.PATH_STAGED, fa.PATH_STAGED_DIRECT):
+        for path in (fa.PATH_AUTO, fa.PATH_FUSED, fa.PATH_STAGED, fa.PATH_STAGED_DIRECT):
             got = run_gpu(torch_cuda, u, shape, boundary=fa.OPEN, rank=rank, lgrid=lgrid,
                           path=path)
             assert_same(got, want, shape)
+
+
+@pytest.mark.parametrize("walk,rows,bpc,seg", [(1, 1, 2, 4), (2, 0, 2, 4), (2, 2, -2, 2), (3, 2, 2, 3),
+                                               (0, 1, -3, 1)])
+def test_rank_positions_open_streaming(torch_cuda, walk, rows, bpc, seg):
+    """Every rank position of a 3x3x3 open decomposition through the one-launch
+    fused path (neighbours as self halos), for each traversal: bit-equal to
+    the staged 26-message form."""
+    shape = (7, 12, 4, 3)
+    n, mx, my, mz = shape
+    print("case rank positions streaming", walk, rows, bpc, seg, flush=True)
+    for rank in range(27):
+        mask = brute.mask_for_rank(rank, 3, 3, 3)
+        u = oracle.new_field(*shape, fa.FILL_RAND, seed=100 + rank)
+        want = u.copy()
+        oracle.share_staged(want, *shape, nbr_mask=mask)
+        d = to_dev(torch_cuda, u)
+        f = fa.Faces(rank, 3, 3, 3, mx, my, mz, n, boundary=fa.OPEN)
+        f.set_path(fa.PATH_FUSED)
+        f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+        f.set_option(fa.OPT_SEG, seg)
+        f.set_option(fa.OPT_STREAM_WALK, walk)
+        f.set_option(fa.OPT_STREAM_ROWS, rows)
+        f.set_option(fa.OPT_STREAM_BPC, bpc)
+        f.share(d)
+        got = to_host(torch_cuda, d)
+        f.close()
+        assert_same(got, want, shape)
 
 
 def test_neighbor_table(torch_cuda):
@@ -382,8 +410,10 @@ def test_invalid_arguments(torch_cuda):
     with pytest.raises(fa.FacesError):
         fa.Faces(0, 1, 1, 1, 2, 2, 2, 4, plane_stride=10)
     f = fa.Faces(0, 2, 1, 1, 2, 2, 2, 4, boundary=fa.OPEN)
+    f.set_path(fa.PATH_FUSED)
+    d = torch_cuda.zeros(fa.field_doubles(4, 2, 2, 2), dtype=torch_cuda.float64, device="cuda")
     with pytest.raises(fa.FacesError):
-        f.set_path(fa.PATH_FUSED)  # rank with a neighbour needs the staged path
+        f.share(d, compute=False)  # surface-only sums of a rank with neighbours: staged path only
     with pytest.raises(fa.FacesError):
         f.set_tuning(100, 0, 0)
     for bad in (-1, -5, 33):
