@@ -15,7 +15,7 @@ B_step = 16 * mx*my*mz*n^3 (the field read once and written once).
 
 Two further legs ride on the same line:
   "batched": 256 independent periodic grids (mx, my, mz uniform in [2,24], n
-  from {4,6,8}, integer-valued [0,1023]) stepped by one persistent launch
+  from {4,6,8}, integer-valued [0,1023]) stepped by one work-list launch
   replayed from a HIP graph after the autotuner picked its shape, against the
   same grids stepped one launch per grid: grids/s, GB/s, and their ratio.
   "served": Poisson arrivals (mean 20k requests/s) of requests sized as the
@@ -143,7 +143,7 @@ def batched_leg(torch, fa, ngrids, steps, warmup, tuning=None):
     del fields
     torch.cuda.synchronize()
     return {"workload": f"{ngrids} periodic grids, mx,my,mz uniform [2,24], n from {{4,6,8}}, "
-                        "integer-valued [0,1023]; one persistent work-list launch per step "
+                        "integer-valued [0,1023]; one work-list launch per step (a chunk per block) "
                         "replayed from a HIP graph",
             "grids": ngrids, "bytes_per_step": int(nbytes),
             "ms_per_step": round(ms_b, 5), "grids_per_s": round(ngrids / (ms_b * 1e-3), 1),

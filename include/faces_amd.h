@@ -102,7 +102,7 @@ int faces_plan_set_tuning(faces_plan *plan, int block, int seg_len, int xcd_rema
 
 /* Kernel of the in-place sum on periodic fields: FACES_VARIANT_DIRECT (one
  * lane per node, copies read straight from HBM), FACES_VARIANT_LDS (element
- * segments staged through LDS), FACES_VARIANT_STREAM (persistent streaming
+ * segments staged through LDS), FACES_VARIANT_STREAM (streaming
  * kernel with item-invariant offset tables), FACES_VARIANT_ITEMS (the
  * batched step's work-list kernel on this one grid: element segments of up
  * to 32 elements, the row's last one shorter, so any mx works); AUTO picks
