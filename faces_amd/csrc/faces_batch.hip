@@ -1,5 +1,5 @@
 // faces_batch.hip -- the batched step: many independent periodic grids (each
-// its own n, mx, my, mz) summed by one persistent launch over a work list of
+// its own n, mx, my, mz) summed by one launch over a work list of
 // element-segment items, replayed from a captured HIP graph.  No counterpart
 // in the reference, which steps one grid per process (gpu/main.cc:170-186).
 #include <algorithm>

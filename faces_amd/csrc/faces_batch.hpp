@@ -36,7 +36,7 @@ int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, 
 // Persistent blocks for a launch: blocks_per_cu per CU, a multiple of kXcds.
 int batch_blocks(int nunits, int blocks_per_cu);
 
-// Enqueues the batched kernel (nblocks persistent blocks, a multiple of kXcds).
+// Enqueues the batched kernel (nblocks blocks, a multiple of kXcds, a chunk each).
 int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t st);
 
 // One periodic grid (n <= 12, field below 4 GB) through the batched kernel:

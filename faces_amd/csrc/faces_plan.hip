@@ -81,11 +81,11 @@ struct faces_plan {
   int block, seg_len, xcd_remap;
   int variant;   // FACES_VARIANT_*
   int lds_block; // threads per block of the LDS-staged kernel
-  int stream_bpc; // persistent blocks per CU of the streaming kernel
+  int stream_bpc; // streaming kernel: persistent blocks per CU (> 0), or one unit per block (<= 0)
   int stream_walk;   // streaming kernel traversal: 0 items, 1 x walk, 2 y walk, 3 row groups
   int stream_rows;   // walk 3: rows per group; walk 2: rows per column piece (the largest
                      // divisor of my not above it); 0 = default per walk
-  int items_bpc;     // work-list item kernel: persistent blocks per CU
+  int items_bpc;     // work-list item kernel: blocks per CU, a chunk of the list each
   int stream_planes; // 1: per-plane descriptors even below 4 GB
   // message arena: send slots then receive slots
   double *arena;

@@ -24,9 +24,10 @@
 // the compiler can wait for exactly the loads it needs (vmcnt on this
 // architecture also counts the preceding item's stores).  The summation
 // touches only the (n-1)^3 - (n-2)^3 shared nodes of each element, whose LDS
-// positions are precomputed.  Blocks are persistent and take items in
-// XCD-contiguous order; each block issues an item's stores and then, at
-// once, the next item's loads.
+// positions are precomputed.  Blocks take work units (rows, column pieces,
+// row groups) in XCD-contiguous order, several each (persistent blocks) or
+// one each; a block issues an item's stores and then, at once, the next
+// item's loads.
 #pragma once
 
 #include <hip/hip_runtime.h>
