@@ -297,7 +297,7 @@ bool stream_ok(const faces_plan *p)
 template <int N, int S>
 int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t st)
 {
-  StreamItems I;
+  StreamItems I = {};
   I.my = p->my;
   I.mz = p->mz;
   I.nseg = p->mx / S;
