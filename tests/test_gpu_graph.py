@@ -20,7 +20,9 @@ CASES = [
     ((7, 12, 5, 3), fa.PERIODIC, 0, (1, 1, 1), fa.PATH_AUTO, 0, True),        # streaming
     ((7, 13, 3, 2), fa.PERIODIC, 0, (1, 1, 1), fa.PATH_AUTO, 0, True),        # work-list items
     ((5, 4, 3, 2), fa.PERIODIC, 0, (1, 1, 1), fa.PATH_AUTO, 0, False),        # surface only
-    ((4, 3, 2, 2), fa.OPEN, 0, (3, 3, 3), fa.PATH_AUTO, 0, True),             # staged, corner rank
+    ((4, 3, 2, 2), fa.OPEN, 0, (3, 3, 3), fa.PATH_AUTO, 0, True),             # open streaming, corner rank
+    ((4, 3, 2, 2), fa.OPEN, 0, (3, 3, 3), fa.PATH_STAGED, 0, True),           # staged, corner rank
+    ((7, 3, 2, 2), fa.OPEN, 4, (3, 3, 3), fa.PATH_STAGED_DIRECT, 0, True),    # staged direct, face rank
     ((4, 3, 2, 2), fa.OPEN, 0, (1, 1, 1), fa.PATH_AUTO, 0, True),             # open 1x1x1
     ((6, 3, 2, 2), fa.PERIODIC, 0, (1, 1, 1), fa.PATH_SIX, 0, True),          # 6-message
     ((13, 2, 2, 2), fa.PERIODIC, 0, (1, 1, 1), fa.PATH_AUTO, 0, True),        # direct (n > 12)
