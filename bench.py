@@ -307,7 +307,7 @@ def main():
         f.share(us[i % nbuf], stream=stream)
     torch.cuda.synchronize()
 
-    if args.no_graph or nbuf > 1:
+    if args.no_graph:
         # Events around every launch on the launch stream: per-launch durations.
         nrep, per_rep = args.steps, 1
         evs = [torch.cuda.Event(enable_timing=True) for _ in range(nrep + 1)]
@@ -320,15 +320,16 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
     else:
-        # A graph of ni shares, replayed; events around every replay.
-        per_rep = max(1, args.ni)
+        # A graph of ni shares (rounded up to whole turns of the rotated
+        # fields), replayed; events around every replay.
+        per_rep = nbuf * max(1, -(-max(1, args.ni) // nbuf))
         nrep = max(1, args.steps // per_rep)
         cap = torch.cuda.Stream()
         torch.cuda.synchronize()
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph, stream=cap):
-            for _ in range(per_rep):
-                f.share(us[0], stream=cap)
+            for k in range(per_rep):
+                f.share(us[k % nbuf], stream=cap)
         torch.cuda.synchronize()
         graph.replay()  # one untimed replay (the first is excluded, as in the reference)
         torch.cuda.synchronize()
@@ -378,8 +379,9 @@ def main():
                    "fields_rotated": nbuf,
                    "cache_resident": bool(nbuf * 8 * total < 512e6),
                    "path": "fused" if args.path in (0, 1) else f"path{args.path}",
-                   "timing": "per launch" if (args.no_graph or nbuf > 1)
-                   else f"graph of {args.ni} shares replayed"},
+                   "timing": "per launch" if args.no_graph
+                   else (f"graph of {per_rep} shares replayed" if nbuf == 1 else
+                         f"graph of {per_rep} shares over the {nbuf} fields in turn, replayed")},
         "steps_per_s": round(1e3 / ms, 1),
         "step_ms_min": round(per_sorted[0], 5),
         "step_ms_p50": round(per_sorted[len(per) // 2], 5),
