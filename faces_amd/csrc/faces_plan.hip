@@ -353,8 +353,10 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t s
     lds_launch = std::max(lds, std::min(share, (size_t)64 * 1024));
   }
   if (mode == 2) {
-    if (big) return fail(FACES_ERR_ARG, "open-rank streaming kernel takes fields under 4 GB");
-    k_stream<N, S, false, true><<<dim3((unsigned)nb), dim3(kStreamThreads), lds_launch, st>>>(G, I);
+    if (big)
+      k_stream<N, S, true, true><<<dim3((unsigned)nb), dim3(kStreamThreads), lds_launch, st>>>(G, I);
+    else
+      k_stream<N, S, false, true><<<dim3((unsigned)nb), dim3(kStreamThreads), lds_launch, st>>>(G, I);
   } else if (big) {
     k_stream<N, S, true><<<dim3((unsigned)nb), dim3(kStreamThreads), lds_launch, st>>>(G, I);
   } else {
@@ -942,7 +944,7 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
   // An open rank takes the fused path where one streaming launch sums it
   // (its neighbours stood in for by self halos), else the staged one.
   const bool open_stream = p->boundary == FACES_BOUNDARY_OPEN && compute &&
-                           active_kernel(p) == FACES_VARIANT_STREAM && !stream_big(p);
+                           active_kernel(p) == FACES_VARIANT_STREAM;
   if (path == FACES_PATH_AUTO)
     path = (p->boundary == FACES_BOUNDARY_PERIODIC || p->mask == 0 || p->mask == 63 || open_stream)
                ? FACES_PATH_FUSED
@@ -975,7 +977,7 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
     if (open_stream) return launch_stream(p, u, 2, s);
     if (p->mask != 0)
       return fail(FACES_ERR_ARG, "fused path of a rank with neighbours needs the streaming kernel "
-                                 "(fields under 4 GB, compute = 1): use the staged path");
+                                 "and compute = 1: use the staged path");
     if (compute) {
       st = launch_box(p, u, true, s);
       if (st) return st;

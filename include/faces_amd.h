@@ -49,8 +49,8 @@ extern "C" {
 #define FACES_PATH_AUTO 0          /* fused when possible, else staged */
 #define FACES_PATH_FUSED 1         /* gather-sum-scatter in place, one launch; for
                                       an open rank with neighbours (self halos)
-                                      it needs the streaming kernel, compute = 1
-                                      and a field under 4 GB */
+                                      it needs the streaming kernel and
+                                      compute = 1 */
 #define FACES_PATH_STAGED 2        /* pack -> exchange copy -> inner + unpack */
 #define FACES_PATH_STAGED_DIRECT 3 /* pack straight into receive slots */
 #define FACES_PATH_SIX 4           /* the 6-message schedule: z, y, x face phases

@@ -146,12 +146,14 @@ def test_open_bitexact(torch_cuda, shape):
 @pytest.mark.parametrize("shape,seg", [((7, 12, 6, 4), 4), ((7, 1, 1, 1), 1), ((5, 10, 3, 2), 5),
                                        ((4, 32, 2, 3), 16), ((8, 3, 4, 5), 3), ((12, 2, 2, 1), 1),
                                        ((2, 4, 1, 3), 4), ((6, 8, 5, 1), 8)])
-def test_open_rank_streaming_bitexact(torch_cuda, walk, rows, bpc, shape, seg):
+@pytest.mark.parametrize("planes", [0, 1])
+def test_open_rank_streaming_bitexact(torch_cuda, walk, rows, bpc, shape, seg, planes):
     """Open boundary on a 1x1x1 rank grid (the reference's own mode) in one
     streaming launch: each rank-surface node sums only the copies on its side
-    of the surface; equal to the open sweep for every traversal."""
+    of the surface; equal to the open sweep for every traversal, with 32-bit
+    field offsets and with the per-plane addressing of fields of 4 GB and more."""
     n, mx, my, mz = shape
-    print("case open streaming", shape, seg, walk, rows, bpc, flush=True)
+    print("case open streaming", shape, seg, walk, rows, bpc, planes, flush=True)
     u = oracle.new_field(*shape, fa.FILL_RAND, seed=808)
     d = to_dev(torch_cuda, u)
     f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.OPEN)
@@ -160,6 +162,7 @@ def test_open_rank_streaming_bitexact(torch_cuda, walk, rows, bpc, shape, seg):
     f.set_option(fa.OPT_STREAM_WALK, walk)
     f.set_option(fa.OPT_STREAM_ROWS, rows)
     f.set_option(fa.OPT_STREAM_BPC, bpc)
+    f.set_option(fa.OPT_STREAM_PLANES, planes)
     assert f.get_option(fa.OPT_ACTIVE_KERNEL) == fa.VARIANT_STREAM
     for _ in range(2):
         f.share(d)
@@ -264,9 +267,10 @@ def test_rank_positions_open(torch_cuda, lgrid):
             assert_same(got, want, shape)
 
 
-@pytest.mark.parametrize("walk,rows,bpc,seg", [(1, 1, 2, 4), (2, 0, 2, 4), (2, 2, -2, 2), (3, 2, 2, 3),
-                                               (0, 1, -3, 1)])
-def test_rank_positions_open_streaming(torch_cuda, walk, rows, bpc, seg):
+@pytest.mark.parametrize("walk,rows,bpc,seg,planes", [(1, 1, 2, 4, 0), (2, 0, 2, 4, 0), (2, 2, -2, 2, 0),
+                                                      (3, 2, 2, 3, 0), (0, 1, -3, 1, 0), (2, 0, 2, 4, 1),
+                                                      (1, 1, -2, 3, 1)])
+def test_rank_positions_open_streaming(torch_cuda, walk, rows, bpc, seg, planes):
     """Every rank position of a 3x3x3 open decomposition through the one-launch
     fused path (neighbours as self halos), for each traversal: bit-equal to
     the staged 26-message form."""
@@ -286,6 +290,7 @@ def test_rank_positions_open_streaming(torch_cuda, walk, rows, bpc, seg):
         f.set_option(fa.OPT_STREAM_WALK, walk)
         f.set_option(fa.OPT_STREAM_ROWS, rows)
         f.set_option(fa.OPT_STREAM_BPC, bpc)
+        f.set_option(fa.OPT_STREAM_PLANES, planes)
         f.share(d)
         got = to_host(torch_cuda, d)
         f.close()
