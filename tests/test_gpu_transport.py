@@ -3,8 +3,8 @@ and field, exchanging their 26 messages through the staged pieces the way an
 external transport would (faces_pack, a device copy of send[d] of each rank
 into recv[opposite d] of its neighbour d, faces_inner, faces_unpack; the MPI
 pattern of gpu/v-26-messages/Faces.cc:144-171, 386-417 with per-direction
-slots).  The ranks together equal the open sweep of the whole grid, bit for
-bit."""
+slots).  The ranks together equal the sweep of the whole grid (open, or
+periodic), bit for bit."""
 from __future__ import annotations
 
 import ctypes
@@ -43,18 +43,23 @@ def whole_grid(fields, lgrid, shape):
     return out
 
 
-@pytest.mark.parametrize("lgrid,shape", [((2, 1, 1), (5, 3, 2, 2)), ((1, 2, 1), (4, 2, 3, 2)),
-                                         ((1, 1, 2), (7, 2, 2, 3)), ((2, 2, 2), (6, 2, 3, 2)),
-                                         ((3, 2, 1), (3, 4, 1, 2))])
-def test_ranks_exchange_through_pieces(torch_cuda, lgrid, shape):
+@pytest.mark.parametrize("lgrid,shape,boundary", [
+    ((2, 1, 1), (5, 3, 2, 2), fa.OPEN), ((1, 2, 1), (4, 2, 3, 2), fa.OPEN),
+    ((1, 1, 2), (7, 2, 2, 3), fa.OPEN), ((2, 2, 2), (6, 2, 3, 2), fa.OPEN),
+    ((3, 2, 1), (3, 4, 1, 2), fa.OPEN),
+    # periodic: with two ranks along an axis both directions lead to the
+    # same neighbour, and the per-direction slots keep the messages apart
+    ((2, 1, 1), (5, 3, 2, 2), fa.PERIODIC), ((2, 2, 2), (4, 2, 2, 2), fa.PERIODIC),
+    ((1, 3, 1), (6, 2, 2, 3), fa.PERIODIC)])
+def test_ranks_exchange_through_pieces(torch_cuda, lgrid, shape, boundary):
     n, mx, my, mz = shape
     lx, ly, lz = lgrid
     nr = lx * ly * lz
-    print("case transport", lgrid, shape, flush=True)
+    print("case transport", lgrid, shape, boundary, flush=True)
     hip = hip_runtime()
     host = [oracle.new_field(*shape, fa.FILL_RAND, seed=50 + r) for r in range(nr)]
     dev = [torch_cuda.from_numpy(h.copy()).cuda() for h in host]
-    plans = [fa.Faces(r, lx, ly, lz, mx, my, mz, n, boundary=fa.OPEN) for r in range(nr)]
+    plans = [fa.Faces(r, lx, ly, lz, mx, my, mz, n, boundary=boundary) for r in range(nr)]
     for p, d in zip(plans, dev):
         p.pack(d)
     torch_cuda.cuda.synchronize()
@@ -78,7 +83,7 @@ def test_ranks_exchange_through_pieces(torch_cuda, lgrid, shape):
     want = whole_grid(host, lgrid, shape)
     gshape = (n, mx * lx, my * ly, mz * lz)
     flat = np.ascontiguousarray(want).reshape(-1)
-    oracle.share_sweep(flat, *gshape, periodic=False)
+    oracle.share_sweep(flat, *gshape, periodic=boundary == fa.PERIODIC)
     got_whole = whole_grid(got, lgrid, shape).reshape(-1)
     r = oracle.compare(got_whole, flat, *gshape)
     assert r["ndiff"] == 0, r
