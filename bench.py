@@ -321,8 +321,12 @@ def main():
         wall = time.perf_counter() - t0
     else:
         # A graph of ni shares (rounded up to whole turns of the rotated
-        # fields), replayed; events around every replay.
+        # fields), replayed; events around every replay.  When the step count
+        # is not a multiple of that, one graph of all the steps, so that
+        # exactly --steps shares are timed.
         per_rep = nbuf * max(1, -(-max(1, args.ni) // nbuf))
+        if args.steps % per_rep:
+            per_rep = max(1, args.steps)
         nrep = max(1, args.steps // per_rep)
         cap = torch.cuda.Stream()
         torch.cuda.synchronize()
