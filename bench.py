@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--path", type=int, default=0, help="FACES_PATH_* (0 auto)")
+    ap.add_argument("--boundary", default="periodic", choices=["periodic", "open"],
+                    help="plain leg on one rank: periodic (default), or open (the reference's own mode)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--seg", type=int, default=-1)
     ap.add_argument("--xcd", type=int, default=-1)
@@ -242,18 +244,18 @@ def inplace_stream_gbs(torch, fa, nbytes, reps=20):
     return gbs
 
 
-def cpu_baseline(n, mx, my, mz, kind, budget_s):
+def cpu_baseline(n, mx, my, mz, kind, budget_s, periodic=True):
     """The oracle's OpenMP sweep on the host cores: same workload, bounded."""
     import numpy as np  # noqa: F401
     import oracle
     u = oracle.new_field(n, mx, my, mz, kind, seed=42)
     t0 = time.perf_counter()
-    oracle.share_sweep(u, n, mx, my, mz, periodic=True)
+    oracle.share_sweep(u, n, mx, my, mz, periodic=periodic)
     first = time.perf_counter() - t0
     reps = max(1, min(50, int(budget_s / max(first, 1e-3))))
     t0 = time.perf_counter()
     for _ in range(reps):
-        oracle.share_sweep(u, n, mx, my, mz, periodic=True)
+        oracle.share_sweep(u, n, mx, my, mz, periodic=periodic)
     dt = (time.perf_counter() - t0) / reps
     b_step = 16.0 * n ** 3 * mx * my * mz
     return {"value": round(b_step / dt / 1e9, 3), "unit": "GB/s",
@@ -292,7 +294,8 @@ def main():
     us = [torch.empty(total, dtype=torch.float64, device="cuda") for _ in range(nbuf)]
     for x in us:
         fa.fill(x, n, mx, my, mz, kind=kind, seed=42)
-    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    periodic = args.boundary == "periodic"
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC if periodic else fa.OPEN)
     f.set_path(args.path)
     if args.block or args.seg >= 0 or args.xcd >= 0:
         f.set_tuning(args.block or 256, max(args.seg, 0), 1 if args.xcd < 0 else args.xcd)
@@ -359,7 +362,7 @@ def main():
     stream_gbs = None if args.no_copy else inplace_stream_gbs(torch, fa, int(8 * total))
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(n, mx, my, mz, kind, args.cpu_seconds)
+        cpu = cpu_baseline(n, mx, my, mz, kind, args.cpu_seconds, periodic)
 
     batched = None if args.no_batch else batched_leg(torch, fa, args.grids, 50, 5, args.batch_tuning)
     served = None if args.no_serve else served_leg(torch, fa, args.requests, args.streams)
@@ -378,8 +381,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": desc, "n": n, "mx": mx, "my": my, "mz": mz,
-                   "rank_grid": "1x1x1 periodic", "bytes_per_step": int(b_step),
+        "config": {"workload": desc if periodic else
+                   desc.replace("periodic", "open (the reference's own mode)", 1),
+                   "n": n, "mx": mx, "my": my, "mz": mz,
+                   "rank_grid": f"1x1x1 {args.boundary}", "bytes_per_step": int(b_step),
                    "fields_rotated": nbuf,
                    "cache_resident": bool(nbuf * 8 * total < 512e6),
                    "path": "fused" if args.path in (0, 1) else f"path{args.path}",
@@ -392,7 +397,7 @@ def main():
         "wall_s": round(wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(args.config)},
+                     "traffic": load_traffic(args.config if periodic else args.config + "_open")},
         "copy_reference_gbs": None if copy_gbs is None else round(copy_gbs, 1),
         "frac_of_copy": None if copy_gbs is None else round(gbs / copy_gbs, 4),
         "inplace_stream_gbs": None if stream_gbs is None else round(stream_gbs, 1),
