@@ -41,9 +41,11 @@ using Clock = std::chrono::steady_clock;
 constexpr double kLargeBytes = 64.0 * 1024 * 1024;
 
 // Blocks per CU of a batched launch (each takes one contiguous chunk of the
-// launch's work list): 4 (2000 requests queued at once: 94k req/s, against
-// 79k with 32, the batched form's default, while two launches overlap).
-constexpr int kServeBlocksPerCu = 4;
+// launch's work list), with two launches in flight: 2 (Poisson 20k req/s:
+// p50 33 us, p99 250 us; 2000 requests queued at once: 98k req/s), against
+// 1 (p99 255 us, 93k req/s), 4 (p99 280-300 us, 94k req/s), 8 (87-88k) and
+// 32 (79k, the batched form's default).
+constexpr int kServeBlocksPerCu = 2;
 
 double now_us(Clock::time_point t0) { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
 
