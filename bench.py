@@ -36,6 +36,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The CPU baseline runs one OpenMP thread per core of the host's CPU share,
+# spread over the cores: the binding is set before any OpenMP runtime starts.
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 METRIC = "HBM GB/s & % roofline per exchange+DSS step; batched grids/s, p99 latency"
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 
@@ -102,6 +107,24 @@ def mixed_shapes(count, seed):
     return out
 
 
+LARGE_SHAPE = (7, 48, 48, 48)  # the served workload's large requests (303.5 MB)
+
+
+def served_workload(nreq, seed_shapes=5, seed_arrivals=20):
+    """Arrival times (us) and shapes (n, mx, my, mz) of the served workload:
+    Poisson arrivals with a mean of 50 us between requests (20k requests/s),
+    sizes as the batched grids, 5% of them 48^3 n=7."""
+    import random
+    rng = random.Random(seed_arrivals)
+    small = mixed_shapes(nreq, seed=seed_shapes)
+    shapes, arrivals, t = [], [], 0.0
+    for i in range(nreq):
+        t += rng.expovariate(1.0 / 50.0)
+        arrivals.append(t)
+        shapes.append(LARGE_SHAPE if rng.random() < 0.05 else small[i])
+    return arrivals, shapes
+
+
 def event_ms(torch, fn, reps, stream):
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -158,15 +181,8 @@ def batched_leg(torch, fa, ngrids, steps, warmup, tuning=None):
 
 
 def served_leg(torch, fa, nreq, args_nstreams=2):
-    import random
     from faces_amd.batch import Server
-    rng = random.Random(20)
-    small = mixed_shapes(nreq, seed=5)
-    shapes, arrivals, t = [], [], 0.0
-    for i in range(nreq):
-        t += rng.expovariate(1.0 / 50.0)  # mean 50 us between arrivals: 20k requests/s
-        arrivals.append(t)
-        shapes.append((7, 48, 48, 48) if rng.random() < 0.05 else small[i])
+    arrivals, shapes = served_workload(nreq)
     fields = []
     for i, sh in enumerate(shapes):
         d = torch.empty(fa.field_doubles(*sh), dtype=torch.float64, device="cuda")
@@ -175,9 +191,11 @@ def served_leg(torch, fa, nreq, args_nstreams=2):
     torch.cuda.synchronize()
     cap = 200e9
     srv = Server(footprint_cap=cap, max_batch=256, nstreams=args_nstreams)
-    # warm the server's launch path on a few requests first
-    warm = srv.run_trace([0.0, 10.0, 20.0], shapes[:3], fields[:3])
-    del warm
+    # warm the server's launch path on a few requests first, one of them
+    # large (its plain-step plan is created when it is first submitted)
+    warm = [0, 1, 2] + [i for i, sh in enumerate(shapes) if sh == LARGE_SHAPE][:1]
+    srv.run_trace([10.0 * k for k in range(len(warm))], [shapes[i] for i in warm],
+                  [fields[i] for i in warm])
     lat, st = srv.run_trace(arrivals, shapes, fields)
     # capacity: every request queued at once
     _, cap_st = srv.run_trace([0.0] * nreq, shapes, fields)
@@ -197,7 +215,7 @@ def served_leg(torch, fa, nreq, args_nstreams=2):
     for f in plans.values():
         f.close()
     nbytes = sum(16.0 * n ** 3 * mx * my * mz for n, mx, my, mz in shapes)
-    nlarge = sum(1 for sh in shapes if sh == (7, 48, 48, 48))
+    nlarge = sum(1 for sh in shapes if sh == LARGE_SHAPE)
     del fields
     torch.cuda.synchronize()
     return {"workload": f"{nreq} requests, Poisson arrivals mean 20k req/s, sizes as the batched "
@@ -244,10 +262,21 @@ def inplace_stream_gbs(torch, fa, nbytes, reps=20):
     return gbs
 
 
+def cpu_threads():
+    """(threads, affinity cpus): one thread per CPU this process may run on,
+    capped by OMP_NUM_THREADS where the host sets it (its CPU share)."""
+    cpus = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    cap = int(env) if env.isdigit() and int(env) > 0 else cpus
+    return min(cpus, cap), cpus
+
+
 def cpu_baseline(n, mx, my, mz, kind, budget_s, periodic=True):
     """The oracle's OpenMP sweep on the host cores: same workload, bounded."""
     import numpy as np  # noqa: F401
     import oracle
+    threads, cpus = cpu_threads()
+    oracle.set_num_threads(threads)
     u = oracle.new_field(n, mx, my, mz, kind, seed=42)
     t0 = time.perf_counter()
     oracle.share_sweep(u, n, mx, my, mz, periodic=periodic)
@@ -258,8 +287,15 @@ def cpu_baseline(n, mx, my, mz, kind, budget_s, periodic=True):
         oracle.share_sweep(u, n, mx, my, mz, periodic=periodic)
     dt = (time.perf_counter() - t0) / reps
     b_step = 16.0 * n ** 3 * mx * my * mz
+    bind = oracle.omp_binding()
     return {"value": round(b_step / dt / 1e9, 3), "unit": "GB/s",
             "cores": oracle.num_threads(), "kind": "port",
+            "nproc": os.cpu_count(), "affinity_cpus": cpus,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+            "binding": f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} "
+                       f"OMP_PLACES={os.environ.get('OMP_PLACES')} (runtime: {bind['proc_bind']}, "
+                       f"{bind['places']} places)",
+            "ms_per_share": round(dt * 1e3, 3),
             "sample": f"{reps} timed shares (+1 untimed) of the full {mx}x{my}x{mz} n={n} field "
                       f"by the oracle's OpenMP pass sweep; {dt * 1e3:.1f} ms/share"}
 

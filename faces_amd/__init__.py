@@ -105,13 +105,21 @@ def _check(status: int, where: str) -> None:
         raise FacesError(status, where, detail)
 
 
-def _addr(u) -> int:
-    """Device address of a torch tensor or an int."""
+def _addr(u, need=None) -> int:
+    """Device address of a torch tensor or an int.  A tensor must be a
+    contiguous float64 GPU tensor of at least ``need`` elements (the kernels
+    address it through raw pointers); a raw address is taken as is."""
     if isinstance(u, int):
         return u
     if hasattr(u, "data_ptr"):
         if not u.is_cuda:
             raise FacesError(ERR_ARG, "field", "tensor is not on the GPU")
+        if torch is not None and u.dtype != torch.float64:
+            raise FacesError(ERR_ARG, "field", f"tensor dtype {u.dtype} is not float64")
+        if not u.is_contiguous():
+            raise FacesError(ERR_ARG, "field", "tensor is not contiguous")
+        if need is not None and u.numel() < need:
+            raise FacesError(ERR_ARG, "field", f"tensor of {u.numel()} elements, the field needs {need}")
         return int(u.data_ptr())
     raise TypeError(f"cannot take a device address of {type(u)!r}")
 
@@ -140,13 +148,15 @@ def stream_ceiling(u, n=None, reps=20, stream=None) -> float:
     if n is None:
         n = u.numel()
     out = ctypes.c_double()
-    _check(_lib.faces_stream_ceiling(_addr(u), n, reps, _stream(stream), ctypes.byref(out)),
+    _check(_lib.faces_stream_ceiling(_addr(u, n), n, reps, _stream(stream), ctypes.byref(out)),
            "faces_stream_ceiling")
     return out.value
 
 
 def fill(u, n, mx, my, mz, kind=FILL_INT, seed=42, plane_stride=0, stream=None) -> None:
-    _check(_lib.faces_fill(_addr(u), n, mx, my, mz, plane_stride, kind, seed, _stream(stream)),
+    need = field_doubles(n, mx, my, mz, plane_stride)
+    _check(_lib.faces_fill(_addr(u, need if need > 0 else None), n, mx, my, mz, plane_stride, kind,
+                           seed, _stream(stream)),
            "faces_fill")
 
 
@@ -163,6 +173,7 @@ class Faces:
         self.rank, self.lx, self.ly, self.lz = rank, lx, ly, lz
         self.mx, self.my, self.mz, self.n = mx, my, mz, n
         self.boundary, self.plane_stride = boundary, plane_stride
+        self.doubles = field_doubles(n, mx, my, mz, plane_stride)
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -182,7 +193,7 @@ class Faces:
         self.close()
 
     def share(self, u, compute=True, stream=None) -> None:
-        _check(_lib.faces_share(self._h, _addr(u), int(bool(compute)), _stream(stream)),
+        _check(_lib.faces_share(self._h, _addr(u, self.doubles), int(bool(compute)), _stream(stream)),
                "faces_share")
 
     def set_path(self, path: int) -> None:
@@ -207,7 +218,8 @@ class Faces:
         """Pick the fastest launch options on this grid (shares u 1 + reps
         times per trial: tune on a scratch copy)."""
         out = (ctypes.c_double * 5)()
-        _check(_lib.faces_plan_tune(self._h, _addr(u), reps, _stream(stream), out), "faces_plan_tune")
+        _check(_lib.faces_plan_tune(self._h, _addr(u, self.doubles), reps, _stream(stream), out),
+               "faces_plan_tune")
         return {"variant": int(out[0]), "walk": int(out[1]), "seg_len": int(out[2]),
                 "blocks_per_cu": int(out[3]), "us_per_share": out[4]}
 
@@ -223,16 +235,16 @@ class Faces:
         return int(p.value or 0), int(c.value)
 
     def pack(self, u, stream=None):
-        _check(_lib.faces_pack(self._h, _addr(u), _stream(stream)), "faces_pack")
+        _check(_lib.faces_pack(self._h, _addr(u, self.doubles), _stream(stream)), "faces_pack")
 
     def exchange(self, stream=None):
         _check(_lib.faces_exchange(self._h, _stream(stream)), "faces_exchange")
 
     def inner(self, u, stream=None):
-        _check(_lib.faces_inner(self._h, _addr(u), _stream(stream)), "faces_inner")
+        _check(_lib.faces_inner(self._h, _addr(u, self.doubles), _stream(stream)), "faces_inner")
 
     def unpack(self, u, stream=None):
-        _check(_lib.faces_unpack(self._h, _addr(u), _stream(stream)), "faces_unpack")
+        _check(_lib.faces_unpack(self._h, _addr(u, self.doubles), _stream(stream)), "faces_unpack")
 
 
 def header_symbols() -> list[str]:

@@ -4,7 +4,7 @@ from __future__ import annotations
 
 import ctypes
 
-from . import FacesError, _addr, _check, _lib, _stream, PERIODIC, OPEN  # noqa: F401
+from . import FacesError, _addr, _check, _lib, _stream, field_doubles, PERIODIC, OPEN  # noqa: F401
 
 _c_int, _c_long, _c_double, _c_u64, _vp = (ctypes.c_int, ctypes.c_long, ctypes.c_double,
                                            ctypes.c_uint64, ctypes.c_void_p)
@@ -69,7 +69,10 @@ class Batch:
             pass
 
     def set_fields(self, fields):
-        ptrs = (_vp * len(fields))(*[_addr(u) for u in fields])
+        if len(fields) != len(self.shapes):
+            raise FacesError(-1, "faces_batch_set_fields",
+                             f"{len(fields)} fields for {len(self.shapes)} grids")
+        ptrs = (_vp * len(fields))(*[_addr(u, field_doubles(*sh)) for u, sh in zip(fields, self.shapes)])
         self._fields = list(fields)  # keep the tensors alive
         _check(_fn("faces_batch_set_fields")(self._h, ptrs), "faces_batch_set_fields")
 
@@ -121,7 +124,8 @@ class Server:
     def submit(self, u, n, mx, my, mz, boundary=PERIODIC, compute=True) -> int:
         t = _c_u64()
         _check(_fn("faces_server_submit")(self._h, n, mx, my, mz, boundary, int(bool(compute)),
-                                          _addr(u), ctypes.byref(t)), "faces_server_submit")
+                                          _addr(u, field_doubles(n, mx, my, mz)), ctypes.byref(t)),
+               "faces_server_submit")
         return t.value
 
     def wait(self, ticket, timeout_s=60.0) -> float:
@@ -134,7 +138,7 @@ class Server:
         nreq = len(arrivals_us)
         arr = (ctypes.c_double * nreq)(*arrivals_us)
         ns, mxs, mys, mzs = zip(*shapes)
-        ptrs = (_vp * nreq)(*[_addr(u) for u in fields])
+        ptrs = (_vp * nreq)(*[_addr(u, field_doubles(*sh)) for u, sh in zip(fields, shapes)])
         lat = (ctypes.c_double * nreq)()
         stats = (ctypes.c_double * 6)()
         _check(_fn("faces_server_run_trace")(self._h, nreq, arr, _ints(ns), _ints(mxs), _ints(mys),

@@ -22,6 +22,14 @@ int hip_fail(hipError_t e, const char *what);
 int device_cus();
 int device_lds_per_cu();
 
+// The staged step (pack into receive slots, interior sums, surface sums) of
+// one rank on a message arena of arena_bytes the caller provides; builds its
+// plan on the host stack and allocates nothing (faces_plan.hip).  `who` names
+// the caller in the error text.
+int share_staged_on_arena(double *u, int rank, int lx, int ly, int lz, int mx, int my, int mz, int n,
+                          int boundary, long plane_stride, int compute, double *arena, size_t arena_bytes,
+                          const char *who, hipStream_t s);
+
 }  // namespace faces
 
 #define FACES_HIP(call)                                          \

@@ -1,0 +1,263 @@
+// faces_flow.hpp -- the one-item-per-block DSS kernel for periodic fields.
+//
+// The work item and its footprint are the streaming kernel's (faces_stream.hpp:
+// S consecutive elements of one element row; OWN planes iz < n-1 of its
+// elements, the Y, Z and W partner rows, the x SEAM column), laid out in LDS
+// as StreamShape<N,S> describes, and every field location is loaded and
+// stored by exactly one item.  What differs is the traversal: every block
+// takes one item, and the items run in memory order -- XCD x (the hardware
+// dispatches block b to XCD b % 8) takes the x-th contiguous eighth of the
+// item list, and its blocks are dispatched in order along it.  At any time
+// the blocks of an XCD work on a compact stretch of the field: an item's
+// x-seam partner (the previous item) and y-partner row (nseg items back) were
+// read and written moments before by the same XCD, so their lines meet again
+// in that XCD's L2, where the partial-line writes of the two items merge
+// before memory sees them.  A block holds no state beyond its item: no
+// carried values, no dummy operations, no loop; short-lived blocks keep many
+// items' loads in flight per CU (up to 8 blocks, set by the registers and
+// LDS a block takes), as a plain streaming copy does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "faces_stream.hpp"
+
+namespace faces {
+
+struct FlowItems {
+  int my, mz;
+  int nseg;              // mx / S
+  int nitems;            // my*mz*nseg
+  unsigned field_bytes;  // below kMaxBufferBytes: offsets are 32-bit
+  int xcd_remap;         // 1: XCD x takes the x-th contiguous eighth; 0: block b takes item b
+};
+
+// W16: the OWN runs are loaded 16 bytes per lane (the field must be 16-byte
+// aligned).  An element's run [b, b + (n-1)n^2) has even length; it takes
+// (n-1)n^2/2 + 1 pair slots, of which an even b uses the first (n-1)n^2/2 and
+// an odd b all, starting one double early and ending one late (the halves
+// outside the run are dropped on the way to LDS).  Stores stay 8 bytes per
+// lane: a pair may hold one location this item owns and one it must not write.
+template <int N, int S, bool W16>
+__global__ __launch_bounds__(kStreamThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_flow(Geom G, FlowItems I)
+{
+  using SH = StreamShape<N, S>;
+  constexpr int n = N, K = N - 1, n2 = N * N;
+  constexpr unsigned n3 = (unsigned)(N * N * N);
+  constexpr int J = SH::J, JO = SH::JO, JP = SH::JP, NN = SH::NN;
+  constexpr unsigned kOOB = kOobBase;
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x;
+
+  // ---- the item: XCD x walks the x-th contiguous eighth of the list
+  int it = blockIdx.x;
+  if (I.xcd_remap) {
+    const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const int q8 = I.nitems >> 3, rem8 = I.nitems & 7;
+    if (k >= q8 + (xcd < rem8 ? 1 : 0)) return;
+    it = xcd * q8 + (xcd < rem8 ? xcd : rem8) + k;
+  }
+  const int sg = it % I.nseg, row = it / I.nseg;
+  const int jy = row % G.my, jz = row / G.my;
+  const int jym = (jy == 0) ? G.my - 1 : jy - 1;
+  const int jzm = (jz == 0) ? G.mz - 1 : jz - 1;
+  const int jx0 = sg * S;
+  const int jxs = (jx0 == 0) ? G.mx - 1 : jx0 - 1;
+  // byte offsets of the four rows: own, y partner, z partner, yz partner
+  const unsigned rb[4] = {(unsigned)((jz * G.ps + jy * G.ys) * 8), (unsigned)((jz * G.ps + jym * G.ys) * 8),
+                          (unsigned)((jzm * G.ps + jy * G.ys) * 8), (unsigned)((jzm * G.ps + jym * G.ys) * 8)};
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(G.u, 0, (int)I.field_bytes, 0x00020000);
+
+  // ---- footprint slots of this thread: load offset, and whether it stores
+  unsigned off[J];
+  unsigned keep = 0;
+#pragma unroll
+  for (int j = 0; j < JO; j++) {
+    const int q = j * kStreamThreads + tid;
+    off[j] = kOOB;
+    if (q < SH::cOWN) {
+      const int e = q / (K * n2), rr = q - e * (K * n2);
+      const int ix = rr % n, iy = (rr / n) % n;
+      off[j] = rb[0] + ((unsigned)(jx0 + e) * n3 + (unsigned)rr) * 8;
+      // x-rows iy = n-1 are the next row's nodes; the last element's ix = n-1
+      // column is the next item's
+      if (iy < K && !(e == S - 1 && ix == K)) keep |= 1u << j;
+    }
+  }
+#pragma unroll
+  for (int jp = 0; jp < JP; jp++) {
+    const int j = JO + jp;
+    int q = jp * kStreamThreads + tid;
+    unsigned o = kOOB;
+    bool kp = false;
+    if (q < SH::cY) {  // row jy-1: x-rows iy = n-1 of planes iz < n-1
+      const int e = q / (K * n), rr = q - e * (K * n);
+      const int iz = rr / n, ix = rr - (rr / n) * n;
+      o = rb[1] + ((unsigned)(jx0 + e) * n3 + (unsigned)(iz * n2 + K * n + ix)) * 8;
+      kp = !(e == S - 1 && ix == K);
+    } else if ((q -= SH::cY) < SH::cZ) {  // row (jy, jz-1): plane iz = n-1, x-rows iy < n-1
+      const int e = q / (K * n), rr = q - e * (K * n);
+      const int ix = rr % n;
+      o = rb[2] + ((unsigned)(jx0 + e) * n3 + (unsigned)(K * n2 + rr)) * 8;
+      kp = !(e == S - 1 && ix == K);
+    } else if ((q -= SH::cZ) < SH::cW) {  // row (jy-1, jz-1): plane iz = n-1, x-row iy = n-1
+      const int e = q / n, ix = q - (q / n) * n;
+      o = rb[3] + ((unsigned)(jx0 + e) * n3 + (unsigned)(K * n2 + K * n + ix)) * 8;
+      kp = !(e == S - 1 && ix == K);
+    } else if ((q -= SH::cW) < SH::cS) {  // the seam: ix = n-1 of element jx0-1 in the four rows
+      kp = true;
+      const unsigned sb = (unsigned)jxs * n3 * 8;
+      if (q < K * K) o = rb[0] + sb + (unsigned)((q / K) * n2 + (q % K) * n + K) * 8;
+      else if (q < K * K + K) o = rb[1] + sb + (unsigned)((q - K * K) * n2 + K * n + K) * 8;
+      else if (q < K * K + 2 * K) o = rb[2] + sb + (unsigned)(K * n2 + (q - K * K - K) * n + K) * 8;
+      else o = rb[3] + sb + (unsigned)(K * n2 + K * n + K) * 8;
+    }
+    off[j] = o;
+    if (kp) keep |= 1u << j;
+  }
+
+  // ---- loads of the whole footprint, all in flight at once
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  constexpr int L = K * n2;               // OWN run of one element (even)
+  constexpr int PE = L / 2 + 1;           // pair slots per element (W16)
+  constexpr int JW = W16 ? (S * PE + kStreamThreads - 1) / kStreamThreads : 1;
+  double v[J];
+  u32x4 w[JW];
+  unsigned wq[JW];  // W16: LDS position of the pair's low half | bit 16: low valid | bit 17: high valid
+  if constexpr (W16) {
+    const unsigned row_d = rb[0] / 8;  // row base in doubles
+#pragma unroll
+    for (int jw = 0; jw < JW; jw++) {
+      const int t = jw * kStreamThreads + tid;
+      unsigned o = kOOB, q = 0;
+      if (t < S * PE) {
+        const int e = t / PE, i = t - (t / PE) * PE;
+        const unsigned eb = (unsigned)(jx0 + e) * n3;  // element base within the row (doubles)
+        const unsigned par = (row_d + eb) & 1u;
+        if (par || i < PE - 1) {
+          o = rb[0] + (eb + 2u * i - par) * 8;
+          q = (unsigned)(e * L + 2 * i) - par;
+          if (!(par && i == 0)) q |= 1u << 16;
+          if (!(par && i == PE - 1)) q |= 1u << 17;
+        }
+      }
+      wq[jw] = q;
+      w[jw] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, o, 0, 0);
+    }
+#pragma unroll
+    for (int j = JO; j < J; j++)
+      v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off[j], 0, 0));
+  } else {
+#pragma unroll
+    for (int j = 0; j < J; j++)
+      v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off[j], 0, 0));
+  }
+
+  // ---- shared-node table (computed while the loads are in flight): the 8
+  // LDS positions of each shared node this thread sums (16 bits each)
+  unsigned np[NN][4];
+  unsigned nflag[NN];  // bit0 cx, bit1 cy, bit2 cz, bit3 active
+#pragma unroll
+  for (int i = 0; i < NN; i++) {
+    const int kk = i * kStreamThreads + tid;
+    nflag[i] = 0;
+    np[i][0] = np[i][1] = np[i][2] = np[i][3] = 0;
+    if (kk >= S * SH::nshared) continue;
+    const int e = kk / SH::nshared;
+    int r = kk - e * SH::nshared;
+    int ix, iy, iz;
+    if (r < K * K) {
+      iz = 0; iy = r / K; ix = r - (r / K) * K;
+    } else if (r < K * K + K * (K - 1)) {
+      r -= K * K;
+      iz = 1 + r / K; iy = 0; ix = r - (r / K) * K;
+    } else {
+      r -= K * K + K * (K - 1);
+      constexpr int K1 = (K > 1) ? K - 1 : 1;  // this branch needs K > 1
+      iz = 1 + r / K1; iy = 1 + r - (r / K1) * K1; ix = 0;
+    }
+    unsigned p[8];
+    p[0] = e * (K * n2) + iz * n2 + iy * n + ix;  // own
+    p[2] = SH::rY + e * (K * n) + iz * n + ix;    // y partner
+    p[4] = SH::rZ + e * (K * n) + iy * n + ix;    // z partner
+    p[6] = SH::rW + e * n + ix;                   // yz partner
+    if (e > 0) {
+      p[1] = (e - 1) * (K * n2) + iz * n2 + iy * n + K;
+      p[3] = SH::rY + (e - 1) * (K * n) + iz * n + K;
+      p[5] = SH::rZ + (e - 1) * (K * n) + iy * n + K;
+      p[7] = SH::rW + (e - 1) * n + K;
+    } else {
+      p[1] = SH::rS + iz * K + iy;
+      p[3] = SH::rS + K * K + iz;
+      p[5] = SH::rS + K * K + K + iy;
+      p[7] = SH::rS + K * K + 2 * K;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; w++) np[i][w] = p[2 * w] | (p[2 * w + 1] << 16);
+    nflag[i] = (ix == 0 ? 1u : 0u) | (iy == 0 ? 2u : 0u) | (iz == 0 ? 4u : 0u) | 8u;
+  }
+
+  if constexpr (W16) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int jw = 0; jw < JW; jw++) {
+      const unsigned q = wq[jw] & 0xffffu;
+      const u32x2 lo = {w[jw][0], w[jw][1]}, hi = {w[jw][2], w[jw][3]};
+      if (wq[jw] & (1u << 16)) lds[q] = __builtin_bit_cast(double, lo);
+      if (wq[jw] & (1u << 17)) lds[q + 1] = __builtin_bit_cast(double, hi);
+    }
+#pragma unroll
+    for (int j = JO; j < J; j++) lds[j * kStreamThreads + tid] = v[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < J; j++) lds[j * kStreamThreads + tid] = v[j];
+  }
+  __syncthreads();
+  // ---- shared nodes: x pairs, then y, then z; the sum goes to every copy
+#pragma unroll
+  for (int i = 0; i < NN; i++) {
+    const unsigned f = nflag[i];
+    if (!(f & 8u)) continue;
+    const unsigned p0 = np[i][0] & 0xffff, p1 = np[i][0] >> 16;
+    const unsigned p2 = np[i][1] & 0xffff, p3 = np[i][1] >> 16;
+    const unsigned p4 = np[i][2] & 0xffff, p5 = np[i][2] >> 16;
+    const unsigned p6 = np[i][3] & 0xffff, p7 = np[i][3] >> 16;
+    const bool cx = f & 1u, cy = f & 2u, cz = f & 4u;
+    const double a0 = cx ? lds[p0] + lds[p1] : lds[p0];
+    double q0 = a0;
+    if (cy) q0 = a0 + (cx ? lds[p2] + lds[p3] : lds[p2]);
+    double res = q0;
+    if (cz) {
+      const double b0 = cx ? lds[p4] + lds[p5] : lds[p4];
+      double q1 = b0;
+      if (cy) q1 = b0 + (cx ? lds[p6] + lds[p7] : lds[p6]);
+      res = q0 + q1;
+    }
+    lds[p0] = res;
+    if (cx) lds[p1] = res;
+    if (cy) {
+      lds[p2] = res;
+      if (cx) lds[p3] = res;
+    }
+    if (cz) {
+      lds[p4] = res;
+      if (cx) lds[p5] = res;
+      if (cy) {
+        lds[p6] = res;
+        if (cx) lds[p7] = res;
+      }
+    }
+  }
+  __syncthreads();
+  // ---- stores of the locations this item owns; the others go out of range
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+  for (int j = 0; j < J; j++) {
+    const u32x2 x = __builtin_bit_cast(u32x2, lds[j * kStreamThreads + tid]);
+    __builtin_amdgcn_raw_buffer_store_b64(x, rsrc, (keep & (1u << j)) ? off[j] : kOOB, 0, 0);
+  }
+}
+
+}  // namespace faces

@@ -22,7 +22,7 @@ namespace faces {
 struct BatchGrid {
   double *u;
   long ys, ps;           // jy and jz strides (doubles)
-  unsigned field_bytes;  // < 2^32 - 8
+  unsigned field_bytes;  // < kMaxBufferBytes
   int n, mx, my, mz;
   int S, nseg;           // elements per item (the row's last item may hold fewer), items per row
   int items;             // my*mz*nseg
@@ -40,7 +40,7 @@ __device__ __forceinline__ int item_offsets(const BatchGrid &G, int it, unsigned
   using SH = StreamShape<N, S>;
   constexpr int n = N, K = N - 1, n2 = N * N, JO = SH::JO;
   constexpr unsigned n3 = (unsigned)(N * N * N);
-  constexpr unsigned kOOB = 0xFFFFFFF8u;
+  constexpr unsigned kOOB = kOobBase;
   const int tid = threadIdx.x;
   const int sg = it % G.nseg;
   const int row = it / G.nseg;
@@ -168,7 +168,7 @@ __device__ __forceinline__ void stream_run(const BatchGrid &G, const Order &item
 {
   using SH = StreamShape<N, S>;
   constexpr int J = SH::J;
-  constexpr unsigned kOOB = 0xFFFFFFF8u;
+  constexpr unsigned kOOB = kOobBase;
   typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
   const int tid = threadIdx.x;
   const __amdgpu_buffer_rsrc_t rsrc =
@@ -181,6 +181,7 @@ __device__ __forceinline__ void stream_run(const BatchGrid &G, const Order &item
     for (int j = 0; j < J; j++)
       v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, ld[j], 0, 0));
   }
+  static_assert(J <= kOobSlots, "dropped stores below the reserved offsets");
 #pragma unroll
   for (int j = 0; j < J; j++)
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, 0.0), rsrc, kOOB - 8 * j, 0, 0);

@@ -11,6 +11,7 @@
 #include "faces_common.hpp"
 #include "faces_kernels.hpp"
 #include "faces_stream.hpp"
+#include "faces_flow.hpp"
 #include "faces_batch.hpp"
 
 namespace faces {
@@ -259,7 +260,8 @@ bool stream_case(int n, int S)
     case 1: case 2: case 3: case 4: return true;
     case 5: return n >= 5 && n <= 8;
     case 10: return n == 5;
-    case 6: case 8: return n <= 6;
+    case 6: return n <= 7;
+    case 8: return n <= 6;
     case 16: return n <= 4;
     case 32: return n == 2;
     default: return false;
@@ -285,18 +287,42 @@ int stream_seg(const faces_plan *p)
 // for fields of 4 GB and more, within one jz plane.
 bool stream_big(const faces_plan *p)
 {
-  return p->stream_planes || 8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps) >= 0xFFFFFFF0L;
+  return p->stream_planes || 8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps) >= kMaxBufferBytes;
 }
 
 bool stream_ok(const faces_plan *p)
 {
   const long plane = 8L * p->n * p->n * p->n * p->mx * p->my;
-  return stream_seg(p) > 0 && plane < 0xFFFFFFF0L;
+  return stream_seg(p) > 0 && plane < kMaxBufferBytes;
 }
 
 template <int N, int S>
 int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t st)
 {
+  if ((p->stream_walk == 5 || p->stream_walk == 6) && mode == 0 && !stream_big(p)) {
+    // one item per block, items in memory order (faces_flow.hpp)
+    FlowItems F;
+    F.my = p->my;
+    F.mz = p->mz;
+    F.nseg = p->mx / S;
+    F.nitems = p->my * p->mz * F.nseg;
+    F.field_bytes = (unsigned)(8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps));
+    F.xcd_remap = p->xcd_remap;
+    const unsigned nb = F.xcd_remap ? 8u * (unsigned)((F.nitems + 7) / 8) : (unsigned)F.nitems;
+    const size_t lds = (size_t)StreamShape<N, S>::P * sizeof(double);
+    size_t lds_launch = lds;
+    if (p->stream_bpc < 0) {  // at most -stream_bpc blocks resident per CU
+      const size_t share = (size_t)device_lds_per_cu() / (size_t)(1 - p->stream_bpc) + 1024;
+      lds_launch = std::max(lds, std::min(share, (size_t)64 * 1024));
+    }
+    // 16-byte OWN loads (walk 6) need a 16-byte aligned field
+    if (p->stream_walk == 6 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0)
+      k_flow<N, S, true><<<dim3(nb), dim3(kStreamThreads), lds_launch, st>>>(G, F);
+    else
+      k_flow<N, S, false><<<dim3(nb), dim3(kStreamThreads), lds_launch, st>>>(G, F);
+    FACES_HIP(hipGetLastError());
+    return FACES_OK;
+  }
   StreamItems I = {};
   I.my = p->my;
   I.mz = p->mz;
@@ -335,6 +361,7 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t s
       while (I.rows > 1 && (long)p->mz * I.nseg * (p->my / I.rows) < want) I.rows = divisor_upto(I.rows - 1);
     }
   }
+  if (I.walk == 4) I.rows = p->stream_rows > 0 ? std::min(p->stream_rows, p->mz) : (p->mz + 7) / 8;
   const long units = (I.walk == 1)   ? (long)p->my * p->mz
                      : (I.walk == 2) ? (long)p->mz * I.nseg * (p->my / I.rows)
                      : (I.walk == 3) ? (long)p->mz * (p->my / I.rows)
@@ -382,8 +409,10 @@ int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, hipStre
   if constexpr (N == 5) {
     if (S == 10) return launch_stream_ns<N, 10>(p, G, mode, st);
   }
-  if constexpr (N <= 6) {
+  if constexpr (N <= 7) {
     if (S == 6) return launch_stream_ns<N, 6>(p, G, mode, st);
+  }
+  if constexpr (N <= 6) {
     if (S == 8) return launch_stream_ns<N, 8>(p, G, mode, st);
   }
   if constexpr (N <= 4) {
@@ -421,7 +450,7 @@ int launch_stream(const faces_plan *p, double *u, int mode, hipStream_t st)
 bool items_ok(const faces_plan *p)
 {
   const long bytes = 8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps);
-  return p->n >= 2 && p->n <= 12 && bytes < 0xFFFFFFF0L && batch_seg(p->n, p->mx, 64) > 0;
+  return p->n >= 2 && p->n <= 12 && bytes < kMaxBufferBytes && batch_seg(p->n, p->mx, 64) > 0;
 }
 
 // The sum kernel faces_share launches on a periodic field with compute = 1.
@@ -573,11 +602,17 @@ long faces_field_doubles(int n, int mx, int my, int mz, long plane_stride)
   return ps * (long)(mz - 1) + dense;
 }
 
-int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx, int my, int mz,
-                      int n, int boundary, long plane_stride)
+}  // extern "C"
+
+namespace {
+
+// Everything of a plan but its device arena: argument checks, rank
+// coordinates, neighbour table, message sizes, launch defaults, and the
+// arena layout (send slots, then receive slots, 512-byte aligned).  Host
+// only: allocates nothing.
+int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, int mz, int n,
+              int boundary, long plane_stride)
 {
-  if (!out) return fail(FACES_ERR_ARG, "null output pointer");
-  *out = nullptr;
   if (n < 2 || n > kMaxN) return fail(FACES_ERR_ARG, "element size n=%d outside [2,%d]", n, kMaxN);
   if (mx < 1 || my < 1 || mz < 1)
     return fail(FACES_ERR_ARG, "element counts must be positive (%d,%d,%d)", mx, my, mz);
@@ -595,9 +630,6 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
   if ((long)mx * (n - 1) + 1 > (1L << 30) || (long)my * (n - 1) + 1 > (1L << 30) ||
       (long)mz * (n - 1) + 1 > (1L << 30) || (long)mx * (n - 1) * (n - 1) * (n - 1) > (1L << 30))
     return fail(FACES_ERR_ARG, "grid too large for one plan");
-
-  faces_plan *p = new (std::nothrow) faces_plan();
-  if (!p) return fail(FACES_ERR_NOMEM, "plan allocation");
   p->rank = rank;
   p->lx = lx; p->ly = ly; p->lz = lz;
   // gpu/v-26-messages/Faces.cc:33-35
@@ -670,14 +702,40 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
     sl.off[26] = off;
   }
   p->arena_doubles = off;
+  p->arena = nullptr;
+  return FACES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx, int my, int mz,
+                      int n, int boundary, long plane_stride)
+{
+  if (!out) return fail(FACES_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  faces_plan *p = new (std::nothrow) faces_plan();
+  if (!p) return fail(FACES_ERR_NOMEM, "plan allocation");
+  int st = plan_init(p, rank, lx, ly, lz, mx, my, mz, n, boundary, plane_stride);
+  if (st) {
+    delete p;
+    return st;
+  }
+  const long off = p->arena_doubles;
   hipError_t e = hipMalloc(&p->arena, sizeof(double) * (size_t)off);
   if (e != hipSuccess) {
     delete p;
     return hip_fail(e, "hipMalloc(message arena)");
   }
-  // Receives from a missing neighbour leave zeros (gpu/DArray.h:81).
-  e = hipMemset(p->arena, 0, sizeof(double) * (size_t)off);
-  if (e == hipSuccess) e = hipDeviceSynchronize();
+  // Receives from a missing neighbour leave zeros (gpu/DArray.h:81).  The
+  // clear runs on a stream of its own and only that stream is waited for, so
+  // creating a plan never waits for other work on the device.
+  hipStream_t zs = nullptr;
+  e = hipStreamCreateWithFlags(&zs, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMemsetAsync(p->arena, 0, sizeof(double) * (size_t)off, zs);
+  if (e == hipSuccess) e = hipStreamSynchronize(zs);
+  if (zs) (void)hipStreamDestroy(zs);
   if (e != hipSuccess) {
     (void)hipFree(p->arena);
     delete p;
@@ -768,12 +826,12 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       p->lds_block = value;
       return FACES_OK;
     case FACES_OPT_STREAM_BPC:
-      if (value < -4 || value == -1 || value > 32)
-        return fail(FACES_ERR_ARG, "blocks per CU %d outside [-4,-2] and [0,32]", value);
+      if (value < -8 || value == -1 || value > 32)
+        return fail(FACES_ERR_ARG, "blocks per CU %d outside [-8,-2] and [0,32]", value);
       p->stream_bpc = value;
       return FACES_OK;
     case FACES_OPT_STREAM_WALK:
-      if (value < 0 || value > 3) return fail(FACES_ERR_ARG, "walk %d outside [0,3]", value);
+      if (value < 0 || value > 6) return fail(FACES_ERR_ARG, "walk %d outside [0,6]", value);
       p->stream_walk = value;
       return FACES_OK;
     case FACES_OPT_STREAM_ROWS:
@@ -1000,6 +1058,49 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
   }
   return launch_surface(p, u, bound_recv(p->mask), s);
 }
+
+}  // extern "C"
+
+namespace faces {
+
+// The staged step -- pack straight into the receive slot of the opposite
+// direction, interior sums, surface sums -- on a message arena the caller
+// provides, with the plan kept on the host stack: nothing is allocated.
+// Only receive slots are laid out (512-byte aligned); every slot the surface
+// sums read is written by the same call's pack (a direction's slot is read
+// only when its neighbour exists), so the arena needs no clearing.
+int share_staged_on_arena(double *u, int rank, int lx, int ly, int lz, int mx, int my, int mz, int n,
+                          int boundary, long plane_stride, int compute, double *arena, size_t arena_bytes,
+                          const char *who, hipStream_t s)
+{
+  if (!u) return fail(FACES_ERR_ARG, "null field pointer");
+  faces_plan p;
+  int st = plan_init(&p, rank, lx, ly, lz, mx, my, mz, n, boundary, plane_stride);
+  if (st) return st;
+  long off = 0;
+  for (int k = 0; k < 26; k++) {
+    p.recv.off[k] = off;
+    off += (p.slot_cnt[k] + 63) / 64 * 64;
+  }
+  p.recv.off[26] = off;
+  p.send = p.recv;  // unused by the direct pack
+  p.arena_doubles = off;
+  if ((size_t)off * sizeof(double) > arena_bytes)
+    return fail(FACES_ERR_ARG, "messages of %ld bytes exceed the %zu-byte arena of %s",
+                off * (long)sizeof(double), arena_bytes, who);
+  p.arena = arena;
+  st = launch_pack(&p, u, true, s);
+  if (st) return st;
+  if (compute) {
+    st = launch_box(&p, u, true, s);
+    if (st) return st;
+  }
+  return launch_surface(&p, u, bound_recv(p.mask), s);
+}
+
+}  // namespace faces
+
+extern "C" {
 
 // In-place read-modify-write stream, one double2 per lane, one launch over
 // the buffer: the bandwidth ceiling of an in-place pass on this device.

@@ -12,6 +12,13 @@
 // it is the same alone or in any mix.  Latency is measured from arrival to the
 // moment the worker observes the launch's completion event.
 //
+// Requests on one field are stepped in submission order: a request whose
+// field is already in a launch (or held back behind another request on it)
+// waits in the queue while later requests on other fields go ahead.  With
+// two or more slots, large requests never take the last free slot, so small
+// requests always have one.  Plans for requests the batched kernel does not
+// take are created by the submitting thread, never inside the worker.
+//
 // No counterpart in the reference, which steps one grid per process.
 #include <hip/hip_runtime.h>
 
@@ -26,6 +33,7 @@
 #include <thread>
 #include <tuple>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "faces_batch.hpp"
@@ -57,7 +65,15 @@ struct Request {
   double t_arrival = 0, t_done = 0;
   bool done = false;
   int status = FACES_OK;
+  faces_plan *plan = nullptr;  // plain-step plan (large or not batchable), set at submission
 };
+
+// Requests the batched kernel takes: periodic, compute = 1, rows with a work
+// item length, 32-bit offsets.  Large requests take the plain step's plan.
+bool batchable(int n, int mx, int boundary, int compute, double bytes)
+{
+  return boundary == FACES_BOUNDARY_PERIODIC && compute && batch_seg(n, mx, 64) > 0 && bytes < (double)kMaxBufferBytes;
+}
 
 struct Slot {
   hipStream_t stream = nullptr;
@@ -66,6 +82,7 @@ struct Slot {
   BatchGrid *h_grids = nullptr;  // pinned staging
   std::vector<Request *> reqs;
   bool busy = false;
+  bool large = false;  // the launch holds one large request
 };
 
 }  // namespace
@@ -84,17 +101,30 @@ struct faces_server {
   uint64_t next_ticket = 1;
   double admitted_bytes = 0, peak_bytes = 0;
   long n_admitted = 0, n_completed = 0, n_launches = 0;
+  int large_busy = 0;                         // slots running a large request
+  std::unordered_set<const double *> fields;  // fields of admitted, unfinished requests
   bool stop = false;
   int worker_error = FACES_OK;
   std::thread worker;
 
-  // plans for requests the batched kernel does not take (open boundary or
-  // compute = 0), keyed by shape; created by the worker on first use
+  // plans for large requests and those the batched kernel does not take
+  // (open boundary, compute = 0), keyed by shape; created by the submitting
+  // thread under plans_mu (never together with mu), destroyed with the server
+  std::mutex plans_mu;
   std::map<std::tuple<int, int, int, int, int>, faces_plan *> plans;
 
   void run();
   int launch(Slot &s, std::vector<Request *> &batch);
+  void release(Request *r, double t);  // under mu
 };
+
+void faces_server::release(Request *r, double t)
+{
+  r->t_done = t;
+  r->done = true;
+  admitted_bytes -= r->bytes;
+  fields.erase(r->u);
+}
 
 int faces_server::launch(Slot &s, std::vector<Request *> &batch)
 {
@@ -105,8 +135,7 @@ int faces_server::launch(Slot &s, std::vector<Request *> &batch)
   // grids than the work-list kernel.
   const bool alone = batch.size() == 1 && batch[0]->bytes >= kLargeBytes;
   for (Request *r : batch) {
-    if (!alone && r->boundary == FACES_BOUNDARY_PERIODIC && r->compute && batch_seg(r->n, r->mx, 64) &&
-        r->bytes < 4294967280.0) {
+    if (!alone && batchable(r->n, r->mx, r->boundary, r->compute, r->bytes)) {
       BatchGrid x;
       x.u = r->u;
       x.n = r->n;
@@ -138,21 +167,15 @@ int faces_server::launch(Slot &s, std::vector<Request *> &batch)
     if (st) return st;
   }
   for (Request *r : other) {
-    auto key = std::make_tuple(r->n, r->mx, r->my, r->mz, r->boundary);
-    faces_plan *&p = plans[key];
-    if (!p) {
-      int st = faces_plan_create(&p, 0, 1, 1, 1, r->mx, r->my, r->mz, r->n, r->boundary, 0);
-      if (st) {
-        p = nullptr;
-        return st;
-      }
-    }
-    int st = faces_share(p, r->u, r->compute, s.stream);
+    if (!r->plan) return fail(FACES_ERR_STATE, "request without a plan");
+    int st = faces_share(r->plan, r->u, r->compute, s.stream);
     if (st) return st;
   }
   FACES_HIP(hipEventRecord(s.event, s.stream));
   s.reqs = batch;
   s.busy = true;
+  s.large = alone;
+  if (alone) large_busy++;
   n_launches++;
   return FACES_OK;
 }
@@ -173,57 +196,63 @@ void faces_server::run()
     for (Slot &s : slots) {
       if (!s.busy) continue;
       const hipError_t q = hipEventQuery(s.event);
-      if (q == hipSuccess) {
-        const double t = now_us(t0);
-        for (Request *r : s.reqs) {
-          r->t_done = t;
-          r->done = true;
-          admitted_bytes -= r->bytes;
-          n_completed++;
-        }
-        s.reqs.clear();
-        s.busy = false;
-        cv_done.notify_all();
-      } else if (q == hipErrorNotReady) {
+      if (q == hipErrorNotReady) {
         any_busy = true;
-      } else {
-        for (Request *r : s.reqs) {
-          r->status = hip_fail(q, "server launch");
-          r->done = true;
-        }
-        s.reqs.clear();
-        s.busy = false;
-        cv_done.notify_all();
+        continue;
       }
+      const double t = now_us(t0);
+      for (Request *r : s.reqs) {
+        if (q != hipSuccess) r->status = hip_fail(q, "server launch");
+        release(r, t);
+        n_completed++;
+      }
+      s.reqs.clear();
+      s.busy = false;
+      if (s.large) large_busy--;
+      s.large = false;
+      cv_done.notify_all();
     }
     // admissions into free slots, in arrival order
     for (Slot &s : slots) {
       if (s.busy || queue.empty()) continue;
       // A launch holds either one large request or small ones only, so small
-      // requests never wait for a large one that shares their launch.
+      // requests never wait for a large one that shares their launch; with
+      // two or more slots, large requests leave the last free slot to small
+      // ones.  A request on a field that an unfinished request holds (or an
+      // earlier request passed over in this scan) stays queued; later
+      // requests on other fields go ahead of it.  A request that does not
+      // fit the footprint cap stops the scan: nothing overtakes it for
+      // footprint.
+      const bool keep_small = slots.size() > 1 && large_busy >= (int)slots.size() - 1;
       std::vector<Request *> batch;
-      while (!queue.empty() && (int)batch.size() < max_batch &&
-             admitted_bytes + queue.front()->bytes <= cap) {
-        Request *r = queue.front();
+      std::unordered_set<const double *> held;
+      for (auto it = queue.begin(); it != queue.end() && (int)batch.size() < max_batch;) {
+        Request *r = *it;
         const bool large = r->bytes >= kLargeBytes;
-        if (large && !batch.empty()) break;
-        queue.pop_front();
+        if (fields.count(r->u) || held.count(r->u) || (large && (keep_small || !batch.empty()))) {
+          held.insert(r->u);
+          ++it;
+          continue;
+        }
+        if (admitted_bytes + r->bytes > cap) break;
+        it = queue.erase(it);
         admitted_bytes += r->bytes;
         peak_bytes = std::max(peak_bytes, admitted_bytes);
+        fields.insert(r->u);
         n_admitted++;
         batch.push_back(r);
         if (large) break;
       }
-      if (batch.empty()) break;  // the head waits for footprint to free up
+      if (batch.empty()) break;  // the queue waits for footprint, a field or a slot
       const int st = launch(s, batch);
       if (st) {
         // part of the batch may already be enqueued: let it finish before
         // the caller learns of the failure (and may free its fields)
         (void)hipStreamSynchronize(s.stream);
+        const double t = now_us(t0);
         for (Request *r : batch) {
           r->status = st;
-          r->done = true;
-          admitted_bytes -= r->bytes;
+          release(r, t);
         }
         cv_done.notify_all();
       } else {
@@ -306,6 +335,25 @@ static int submit_at(faces_server *s, int n, int mx, int my, int mz, int boundar
     return fail(FACES_ERR_ARG, "unknown boundary %d", boundary);
   const double bytes = 8.0 * faces_field_doubles(n, mx, my, mz, 0);
   if (bytes > s->cap) return fail(FACES_ERR_ADMISSION, "request of %.0f bytes exceeds the cap %.0f", bytes, s->cap);
+  // The plain-step plan of a large or non-batchable request is looked up, or
+  // created on the server's device, here, by the submitting thread.
+  faces_plan *plan = nullptr;
+  if (bytes >= kLargeBytes || !batchable(n, mx, boundary, compute, bytes)) {
+    std::lock_guard<std::mutex> lk(s->plans_mu);
+    faces_plan *&p = s->plans[std::make_tuple(n, mx, my, mz, boundary)];
+    if (!p) {
+      int dev0 = 0;
+      if (hipGetDevice(&dev0) != hipSuccess) return fail(FACES_ERR_HIP, "hipGetDevice");
+      if (dev0 != s->device && hipSetDevice(s->device) != hipSuccess) return fail(FACES_ERR_HIP, "hipSetDevice");
+      const int st = faces_plan_create(&p, 0, 1, 1, 1, mx, my, mz, n, boundary, 0);
+      if (dev0 != s->device) (void)hipSetDevice(dev0);
+      if (st) {
+        p = nullptr;
+        return st;
+      }
+    }
+    plan = p;
+  }
   Request *r = new (std::nothrow) Request();
   if (!r) return fail(FACES_ERR_NOMEM, "request allocation");
   r->n = n;
@@ -316,6 +364,7 @@ static int submit_at(faces_server *s, int n, int mx, int my, int mz, int boundar
   r->compute = compute ? 1 : 0;
   r->u = u;
   r->bytes = bytes;
+  r->plan = plan;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) {

@@ -39,6 +39,16 @@ namespace faces {
 
 constexpr int kStreamThreads = 256;
 
+// Buffer-addressed kernels use 32-bit byte offsets.  Offsets from kOobBase
+// down to kOobBase - 8 * kOobSlots are sent out of range on purpose (loads and
+// stores the hardware drops); a field, or in per-plane addressing one plane,
+// is addressed this way only when its bytes stay below kMaxBufferBytes, so
+// none of those offsets can reach it.
+constexpr unsigned kOobBase = 0xFFFFFFF8u;
+constexpr int kOobSlots = 64;
+constexpr long kMaxBufferBytes = 0xFFFFF000L;
+static_assert((long)kOobBase - 8L * kOobSlots >= kMaxBufferBytes, "dropped offsets must lie above every field");
+
 template <int N, int S>
 struct StreamShape {
   static constexpr int n = N, K = N - 1, n2 = N * N, n3 = N * N * N;
@@ -58,7 +68,7 @@ struct StreamItems {
   int my, mz;
   int nseg;             // mx / S
   int nitems;           // my*mz*nseg
-  unsigned field_bytes; // below 2^32 - 8: offsets are 32-bit (BIG kernels: one plane's bytes)
+  unsigned field_bytes; // below kMaxBufferBytes: offsets are 32-bit (BIG kernels: one plane's bytes)
   int xcd_remap;        // 1: contiguous ranges per XCD
   int walk;             // 0 items, 1 rows in x (seam carried), 2 columns in y (y rows carried),
                         // 3 groups of `rows` rows, segment by segment (seam and y rows carried)
@@ -241,7 +251,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
   const int units = (I.walk == 1) ? I.my * I.mz
                     : (I.walk == 2) ? I.mz * I.nseg * (I.my / L)
                     : (I.walk == 3) ? I.mz * (I.my / R)
-                                    : I.nitems;
+                                    : I.nitems;  // walk 0, walk 4
   int start, cnt, k, nbx;
   if (I.xcd_remap) {
     const int xcd = blockIdx.x & 7;
@@ -265,6 +275,16 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     // walk 2: c = (jz * (my / L) + piece) * nseg + segment; its rows start at
     // jz * my + piece * L = (c / nseg) * L
     if (I.walk == 2) return ((c / I.nseg) * L + ss) * I.nseg + (c % I.nseg);
+    if (I.walk == 4) {
+      // bands of B = I.rows planes; within a band, rows in y, the band's
+      // planes of one y innermost: the z and y partners of an item come
+      // nseg and B*nseg items before it
+      const int B = I.rows, seg = c % I.nseg, r = c / I.nseg;
+      const int b = r / (B * I.my), rr = r - b * (B * I.my);
+      const int d = min(B, I.mz - b * B);
+      const int jy = rr / d, jz = b * B + (rr - (rr / d) * d);
+      return (jz * I.my + jy) * I.nseg + seg;
+    }
     return c;
   };
   double *const ring = lds + SH::P;  // saved seam columns, one per row of the group
@@ -275,7 +295,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
   // instruction is unconditional, and the compiler waits on exact counts.
   const __amdgpu_buffer_rsrc_t rsrc =
       __builtin_amdgcn_make_buffer_rsrc(G.u, 0, (int)I.field_bytes, 0x00020000);
-  constexpr unsigned kOOB = 0xFFFFFFF8u;  // any offset at or above field_bytes is dropped
+  constexpr unsigned kOOB = kOobBase;  // any offset at or above field_bytes is dropped
 #pragma unroll
   for (int j = 0; j < J; j++) voff[j] *= 8;  // bytes from here on
   unsigned dropm[J];
@@ -353,6 +373,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
   // iteration -- an item's loads, then its stores -- so the waits at the loop
   // head are exact.
   constexpr int NST = J + (BIG ? SH::JP : 0);
+  static_assert(NST <= kOobSlots, "dropped stores below the reserved offsets");
 #pragma unroll
   for (int j = 0; j < NST; j++)
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, 0.0), rsrc, kOOB - 8 * j, 0, 0);

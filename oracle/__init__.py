@@ -49,6 +49,10 @@ def lib() -> ctypes.CDLL:
         L.orc_opposite_slot.restype = i
         L.orc_num_threads.argtypes = []
         L.orc_num_threads.restype = i
+        L.orc_set_num_threads.argtypes = [i]
+        L.orc_set_num_threads.restype = None
+        L.orc_omp_binding.argtypes = [ctypes.POINTER(i)]
+        L.orc_omp_binding.restype = None
         _lib = L
     return _lib
 
@@ -126,3 +130,15 @@ def opposite_slot(k: int) -> int:
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+def set_num_threads(t: int) -> None:
+    lib().orc_set_num_threads(int(t))
+
+
+def omp_binding() -> dict:
+    """The OpenMP runtime's thread binding: proc_bind name and place count."""
+    out = (ctypes.c_int * 2)()
+    lib().orc_omp_binding(out)
+    names = {0: "false", 1: "true", 2: "master", 3: "close", 4: "spread"}
+    return {"proc_bind": names.get(out[0], str(out[0])), "places": int(out[1])}

@@ -634,3 +634,27 @@ int orc_num_threads(void)
   return 1;
 #endif
 }
+
+/* Threads of the following parallel regions (the CPU baseline sets one per
+ * core of the host's CPU share). */
+void orc_set_num_threads(int t)
+{
+#ifdef _OPENMP
+  if (t > 0) omp_set_num_threads(t);
+#else
+  (void)t;
+#endif
+}
+
+/* OpenMP binding as the runtime sees it: out[0] = omp_get_proc_bind()
+ * (0 false, 1 true, 2 master, 3 close, 4 spread), out[1] = places. */
+void orc_omp_binding(int *out)
+{
+#ifdef _OPENMP
+  out[0] = (int)omp_get_proc_bind();
+  out[1] = omp_get_num_places();
+#else
+  out[0] = 0;
+  out[1] = 0;
+#endif
+}

@@ -127,7 +127,7 @@ def test_batch_graph_replays(torch_cuda):
 
 def test_batch_matches_per_grid_256(torch_cuda):
     """256 mixed grids (mx, my, mz in [2,24], n in {4,6,8}): batched launch
-    equals one launch per grid on all of them, and the oracle on a subset."""
+    equals one launch per grid and the oracle on every one of them."""
     print("case batch 256 mixed", flush=True)
     shapes = mixed_shapes(256, seed=2024)
     a = make_fields(torch_cuda, shapes, seed0=5000)
@@ -140,9 +140,7 @@ def test_batch_matches_per_grid_256(torch_cuda):
     ga, gc = host(torch_cuda, a), host(torch_cuda, c)
     for i in range(256):
         assert np.array_equal(ga[i].view(np.uint64), gc[i].view(np.uint64)), (i, shapes[i])
-    sub = list(range(0, 256, 16))
-    assert_all_same([ga[i] for i in sub], [oracle_one(shapes[i], 5000 + i) for i in sub],
-                    [shapes[i] for i in sub])
+    assert_all_same(ga, [oracle_one(shapes[i], 5000 + i) for i in range(256)], shapes)
     b1.close()
     b2.close()
 

@@ -228,3 +228,47 @@ def test_large_requests_launch_alone(torch_cuda):
         r = oracle.compare(fields[i].cpu().numpy(), oracle_one(s, 500 + i), *s)
         assert r["ndiff"] == 0, (s, r)
     srv.close()
+
+
+def test_served_workload_full_size(torch_cuda):
+    """The served workload at full size: 600 requests, Poisson arrivals at
+    20k requests/s, mx, my, mz uniform in [2,24], n from {4,6,8}, 5% of them
+    48^3 n=7, random FP64 [-1,1), footprint cap 200 GB, two launch slots.
+    Every large request and every 8th small one equals the oracle bit for
+    bit; three probes (two small, one large) give the same bits alone and
+    inside the trace."""
+    from bench import LARGE_SHAPE, served_workload
+    print("case serve full-size workload", flush=True)
+    nreq = 600
+    arrivals, shapes = served_workload(nreq)
+    large = [i for i, s in enumerate(shapes) if s == LARGE_SHAPE]
+    small = [i for i, s in enumerate(shapes) if s != LARGE_SHAPE]
+    assert len(large) >= 15, len(large)
+    assert all(s[0] in (4, 6, 8) and all(2 <= m <= 24 for m in s[1:]) for s in (shapes[i] for i in small))
+    probes = [small[len(small) // 4], large[len(large) // 2], small[3 * len(small) // 4]]
+    seed = lambda i: 9000 + i  # noqa: E731
+    srv = Server(footprint_cap=200e9, max_batch=256, nstreams=2)
+    alone = []
+    for i in probes:
+        d = field(torch_cuda, shapes[i], seed(i))
+        torch_cuda.cuda.synchronize()
+        srv.wait(srv.submit(d, *shapes[i]))
+        alone.append(d.cpu().numpy())
+        del d
+    fields = [field(torch_cuda, s, seed(i)) for i, s in enumerate(shapes)]
+    torch_cuda.cuda.synchronize()
+    lat, st = srv.run_trace(arrivals, shapes, fields)
+    assert len(lat) == nreq and min(lat) > 0
+    assert st["p50_us"] <= st["p99_us"] <= st["max_us"]
+    assert srv.stats()["peak_bytes"] <= 200e9
+    print(f"served {nreq} requests ({len(large)} large): p50 {st['p50_us']:.1f} us, "
+          f"p99 {st['p99_us']:.1f} us, {st['launches']:.0f} launches", flush=True)
+    srv.close()
+    for j, i in enumerate(probes):
+        got = fields[i].cpu().numpy()
+        assert np.array_equal(got.view(np.uint64), alone[j].view(np.uint64)), (i, shapes[i])
+    checked = large + small[::8]
+    for i in checked:
+        r = oracle.compare(fields[i].cpu().numpy(), oracle_one(shapes[i], seed(i)), *shapes[i])
+        assert r["ndiff"] == 0, (i, shapes[i], r)
+    print(f"checked {len(checked)} requests against the oracle", flush=True)
