@@ -36,10 +36,6 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# The CPU baseline runs one OpenMP thread per core of the host's CPU share,
-# spread over the cores: the binding is set before any OpenMP runtime starts.
-os.environ.setdefault("OMP_PROC_BIND", "spread")
-os.environ.setdefault("OMP_PLACES", "cores")
 
 METRIC = "HBM GB/s & % roofline per exchange+DSS step; batched grids/s, p99 latency"
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
@@ -85,6 +81,7 @@ def parse():
                     help="batched leg launch shape SEG,BLOCKS_PER_CU instead of the tuner's")
     ap.add_argument("--requests", type=int, default=2000, help="requests in the served leg")
     ap.add_argument("--streams", type=int, default=2, help="launch slots of the served leg")
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -271,12 +268,11 @@ def cpu_threads():
     return min(cpus, cap), cpus
 
 
-def cpu_baseline(n, mx, my, mz, kind, budget_s, periodic=True):
-    """The oracle's OpenMP sweep on the host cores: same workload, bounded."""
-    import numpy as np  # noqa: F401
+def cpu_child(spec):
+    """Runs in a child process with its OpenMP threads bound (see
+    cpu_baseline): times the oracle's sweep and prints one JSON line."""
     import oracle
-    threads, cpus = cpu_threads()
-    oracle.set_num_threads(threads)
+    n, mx, my, mz, kind, budget_s, periodic = json.loads(spec)
     u = oracle.new_field(n, mx, my, mz, kind, seed=42)
     t0 = time.perf_counter()
     oracle.share_sweep(u, n, mx, my, mz, periodic=periodic)
@@ -286,17 +282,32 @@ def cpu_baseline(n, mx, my, mz, kind, budget_s, periodic=True):
     for _ in range(reps):
         oracle.share_sweep(u, n, mx, my, mz, periodic=periodic)
     dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"dt": dt, "reps": reps, "threads": oracle.num_threads(),
+                      "binding": oracle.omp_binding()}), flush=True)
+
+
+def cpu_baseline(n, mx, my, mz, kind, budget_s, periodic=True):
+    """The oracle's OpenMP sweep on the host cores: same workload, bounded.
+    It runs in a child process whose OpenMP threads, one per core of this
+    process's CPU share, are bound spread over the cores; binding them here
+    would pin this process (and the server's threads) to one core."""
+    import subprocess
+    threads, cpus = cpu_threads()
+    env = dict(os.environ, OMP_PROC_BIND="spread", OMP_PLACES="cores", OMP_NUM_THREADS=str(threads))
+    spec = json.dumps([n, mx, my, mz, kind, budget_s, bool(periodic)])
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", spec], env=env,
+                         capture_output=True, text=True, check=True, timeout=600)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    dt = r["dt"]
     b_step = 16.0 * n ** 3 * mx * my * mz
-    bind = oracle.omp_binding()
     return {"value": round(b_step / dt / 1e9, 3), "unit": "GB/s",
-            "cores": oracle.num_threads(), "kind": "port",
+            "cores": r["threads"], "kind": "port",
             "nproc": os.cpu_count(), "affinity_cpus": cpus,
             "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
-            "binding": f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} "
-                       f"OMP_PLACES={os.environ.get('OMP_PLACES')} (runtime: {bind['proc_bind']}, "
-                       f"{bind['places']} places)",
+            "binding": f"OMP_PROC_BIND=spread OMP_PLACES=cores (runtime: {r['binding']['proc_bind']}, "
+                       f"{r['binding']['places']} places)",
             "ms_per_share": round(dt * 1e3, 3),
-            "sample": f"{reps} timed shares (+1 untimed) of the full {mx}x{my}x{mz} n={n} field "
+            "sample": f"{r['reps']} timed shares (+1 untimed) of the full {mx}x{my}x{mz} n={n} field "
                       f"by the oracle's OpenMP pass sweep; {dt * 1e3:.1f} ms/share"}
 
 
@@ -314,6 +325,9 @@ def load_traffic(name):
 
 def main():
     args = parse()
+    if args.cpu_child:
+        cpu_child(args.cpu_child)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
               file=sys.stderr)

@@ -22,6 +22,10 @@ int hip_fail(hipError_t e, const char *what);
 int device_cus();
 int device_lds_per_cu();
 
+// The current device's stream for the library's set-up work (non-blocking,
+// created on first use).
+hipError_t setup_stream(hipStream_t *out);
+
 // The staged step (pack into receive slots, interior sums, surface sums) of
 // one rank on a message arena of arena_bytes the caller provides; builds its
 // plan on the host stack and allocates nothing (faces_plan.hip).  `who` names

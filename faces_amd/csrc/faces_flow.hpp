@@ -30,7 +30,9 @@ struct FlowItems {
   int nseg;              // mx / S
   int nitems;            // my*mz*nseg
   unsigned field_bytes;  // below kMaxBufferBytes: offsets are 32-bit
-  int xcd_remap;         // 1: XCD x takes the x-th contiguous eighth; 0: block b takes item b
+  int xcd_remap;         // 2: XCD x takes the x-th eighth of every plane's items, planes in
+                         // order; 1: XCD x takes the x-th contiguous eighth of the list;
+                         // 0: block b takes item b
 };
 
 // W16: the OWN runs are loaded 16 bytes per lane (the field must be 16-byte
@@ -51,9 +53,19 @@ void k_flow(Geom G, FlowItems I)
   extern __shared__ double lds[];
   const int tid = threadIdx.x;
 
-  // ---- the item: XCD x walks the x-th contiguous eighth of the list
+  // ---- the item
   int it = blockIdx.x;
-  if (I.xcd_remap) {
+  if (I.xcd_remap == 2) {
+    // The x-th eighth of the items of every plane, planes in order: an
+    // item's z partner is one plane's share (an eighth of a plane) back in
+    // this XCD's walk, its y partner nseg items back.
+    const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const int per = G.my * I.nseg;  // items per plane
+    const int i0 = xcd * per / 8, cnt = (xcd + 1) * per / 8 - i0;
+    if (cnt == 0 || k >= cnt * G.mz) return;
+    const int jzk = k / cnt;
+    it = jzk * per + i0 + (k - jzk * cnt);
+  } else if (I.xcd_remap == 1) {
     const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
     const int q8 = I.nitems >> 3, rem8 = I.nitems & 7;
     if (k >= q8 + (xcd < rem8 ? 1 : 0)) return;
@@ -125,7 +137,7 @@ void k_flow(Geom G, FlowItems I)
   constexpr int JW = W16 ? (S * PE + kStreamThreads - 1) / kStreamThreads : 1;
   double v[J];
   u32x4 w[JW];
-  unsigned wq[JW];  // W16: LDS position of the pair's low half | bit 16: low valid | bit 17: high valid
+  unsigned wq[JW];  // W16: LDS position of the pair's high half | bit 16: low valid | bit 17: high valid
   if constexpr (W16) {
     const unsigned row_d = rb[0] / 8;  // row base in doubles
 #pragma unroll
@@ -138,7 +150,7 @@ void k_flow(Geom G, FlowItems I)
         const unsigned par = (row_d + eb) & 1u;
         if (par || i < PE - 1) {
           o = rb[0] + (eb + 2u * i - par) * 8;
-          q = (unsigned)(e * L + 2 * i) - par;
+          q = (unsigned)(e * L + 2 * i + 1) - par;  // >= 0: the high half lies in the run or just past it
           if (!(par && i == 0)) q |= 1u << 16;
           if (!(par && i == PE - 1)) q |= 1u << 17;
         }
@@ -205,8 +217,8 @@ void k_flow(Geom G, FlowItems I)
     for (int jw = 0; jw < JW; jw++) {
       const unsigned q = wq[jw] & 0xffffu;
       const u32x2 lo = {w[jw][0], w[jw][1]}, hi = {w[jw][2], w[jw][3]};
-      if (wq[jw] & (1u << 16)) lds[q] = __builtin_bit_cast(double, lo);
-      if (wq[jw] & (1u << 17)) lds[q + 1] = __builtin_bit_cast(double, hi);
+      if (wq[jw] & (1u << 16)) lds[q - 1] = __builtin_bit_cast(double, lo);
+      if (wq[jw] & (1u << 17)) lds[q] = __builtin_bit_cast(double, hi);
     }
 #pragma unroll
     for (int j = JO; j < J; j++) lds[j * kStreamThreads + tid] = v[j];

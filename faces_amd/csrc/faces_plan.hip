@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -67,6 +68,29 @@ int device_lds_per_cu()
   return v;
 }
 
+// A non-blocking stream per device (created on first use, kept until the
+// process ends) for the library's set-up work, so that set-up waits only for
+// itself.
+hipError_t setup_stream(hipStream_t *out)
+{
+  static std::mutex mu;
+  static hipStream_t streams[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!streams[dev]) {
+    e = hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      streams[dev] = nullptr;
+      return e;
+    }
+  }
+  *out = streams[dev];
+  return hipSuccess;
+}
+
 }  // namespace faces
 
 using namespace faces;
@@ -88,6 +112,8 @@ struct faces_plan {
                      // divisor of my not above it); 0 = default per walk
   int items_bpc;     // work-list item kernel: blocks per CU, a chunk of the list each
   int stream_planes; // 1: per-plane descriptors even below 4 GB
+  int flow;          // periodic steps with compute = 1: 0 the streaming kernel; 1 the one-item
+                     // kernel (faces_flow.hpp), 8-byte lanes; 2 the same with 16-byte OWN loads
   // message arena: send slots then receive slots
   double *arena;
   long arena_doubles;
@@ -283,6 +309,21 @@ int stream_seg(const faces_plan *p)
   return 0;
 }
 
+// Elements per item of the one-item kernel: as stream_seg, with its own
+// defaults per n (60^3 n=7: 5 elements 249-253 us, 4 elements 254-256 us,
+// 6 elements 251-252 us).
+int flow_seg(const faces_plan *p)
+{
+  if (p->n < 2 || p->n > 12) return 0;
+  static const int kDefault[13] = {0, 0, 32, 16, 16, 10, 5, 5, 4, 3, 2, 1, 1};
+  const int want = p->seg_len > 0 ? p->seg_len : kDefault[p->n];
+  for (int s = std::min(want, 32); s >= 1; s--) {
+    if (!stream_case(p->n, s) || p->mx % s) continue;
+    if (stream_slots(p->n, s) <= kStreamMaxSlots) return s;
+  }
+  return 0;
+}
+
 // Byte offsets of the streaming kernel are 32-bit: within the field, or,
 // for fields of 4 GB and more, within one jz plane.
 bool stream_big(const faces_plan *p)
@@ -299,7 +340,7 @@ bool stream_ok(const faces_plan *p)
 template <int N, int S>
 int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t st)
 {
-  if ((p->stream_walk == 5 || p->stream_walk == 6) && mode == 0 && !stream_big(p)) {
+  if (p->flow && mode == 0 && !stream_big(p)) {
     // one item per block, items in memory order (faces_flow.hpp)
     FlowItems F;
     F.my = p->my;
@@ -308,18 +349,21 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t s
     F.nitems = p->my * p->mz * F.nseg;
     F.field_bytes = (unsigned)(8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps));
     F.xcd_remap = p->xcd_remap;
-    const unsigned nb = F.xcd_remap ? 8u * (unsigned)((F.nitems + 7) / 8) : (unsigned)F.nitems;
-    const size_t lds = (size_t)StreamShape<N, S>::P * sizeof(double);
-    size_t lds_launch = lds;
-    if (p->stream_bpc < 0) {  // at most -stream_bpc blocks resident per CU
-      const size_t share = (size_t)device_lds_per_cu() / (size_t)(1 - p->stream_bpc) + 1024;
-      lds_launch = std::max(lds, std::min(share, (size_t)64 * 1024));
+    unsigned nb = (unsigned)F.nitems;
+    if (F.xcd_remap == 2) {  // the XCD with the largest share of a plane sets the grid
+      const int per = p->my * F.nseg;
+      int cmax = 0;
+      for (int x = 0; x < 8; x++) cmax = std::max(cmax, (x + 1) * per / 8 - x * per / 8);
+      nb = 8u * (unsigned)(cmax * p->mz);
+    } else if (F.xcd_remap == 1) {
+      nb = 8u * (unsigned)((F.nitems + 7) / 8);
     }
-    // 16-byte OWN loads (walk 6) need a 16-byte aligned field
-    if (p->stream_walk == 6 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0)
-      k_flow<N, S, true><<<dim3(nb), dim3(kStreamThreads), lds_launch, st>>>(G, F);
+    const size_t lds = (size_t)StreamShape<N, S>::P * sizeof(double);
+    // 16-byte OWN loads need a 16-byte aligned field
+    if (p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0)
+      k_flow<N, S, true><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
     else
-      k_flow<N, S, false><<<dim3(nb), dim3(kStreamThreads), lds_launch, st>>>(G, F);
+      k_flow<N, S, false><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
     FACES_HIP(hipGetLastError());
     return FACES_OK;
   }
@@ -361,7 +405,6 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t s
       while (I.rows > 1 && (long)p->mz * I.nseg * (p->my / I.rows) < want) I.rows = divisor_upto(I.rows - 1);
     }
   }
-  if (I.walk == 4) I.rows = p->stream_rows > 0 ? std::min(p->stream_rows, p->mz) : (p->mz + 7) / 8;
   const long units = (I.walk == 1)   ? (long)p->my * p->mz
                      : (I.walk == 2) ? (long)p->mz * I.nseg * (p->my / I.rows)
                      : (I.walk == 3) ? (long)p->mz * (p->my / I.rows)
@@ -427,7 +470,8 @@ int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, hipStre
 int launch_stream(const faces_plan *p, double *u, int mode, hipStream_t st)
 {
   const Geom G = make_geom(p, u);
-  const int S = stream_seg(p);
+  const int S = (p->flow && mode == 0 && !stream_big(p)) ? flow_seg(p) : stream_seg(p);
+  if (S == 0) return fail(FACES_ERR_ARG, "no streaming item length for n=%d mx=%d", p->n, p->mx);
   switch (p->n) {
     case 2: return launch_stream_n<2>(p, G, S, mode, st);
     case 3: return launch_stream_n<3>(p, G, S, mode, st);
@@ -463,8 +507,10 @@ int active_kernel(const faces_plan *p)
     case FACES_VARIANT_ITEMS: return items_ok(p) ? FACES_VARIANT_ITEMS : FACES_VARIANT_DIRECT;
     default:
       // the work-list kernel only where its items (ragged rows allowed) are
-      // more than twice as long as the streaming kernel's
-      if (stream_ok(p) && !(items_ok(p) && batch_seg(p->n, p->mx, 64) > 2 * stream_seg(p)))
+      // more than twice as long as the streaming kernel's (the one-item
+      // kernel's, where periodic steps take it)
+      if (stream_ok(p) &&
+          !(items_ok(p) && batch_seg(p->n, p->mx, 64) > 2 * (p->flow && !stream_big(p) ? flow_seg(p) : stream_seg(p))))
         return FACES_VARIANT_STREAM;
       if (items_ok(p)) return FACES_VARIANT_ITEMS;
       return stream_ok(p) ? FACES_VARIANT_STREAM : FACES_VARIANT_DIRECT;
@@ -650,7 +696,11 @@ int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, i
   p->path = FACES_PATH_AUTO;
   p->block = 256;
   p->seg_len = 0;
-  p->xcd_remap = 1;
+  // XCD order: each XCD takes an eighth of every plane's items (the one-item
+  // kernel; 64^3 n=8 410 us against 423 for contiguous eighths of the list,
+  // 80^3 n=6 356 against 397); the other kernels read it as 1, contiguous
+  // eighths of their work list.
+  p->xcd_remap = 2;
   p->variant = FACES_VARIANT_AUTO;
   p->lds_block = 256;
   // Blocks: 2 persistent blocks per CU for n = 6-7 (60^3 n=7 276 us against
@@ -677,6 +727,10 @@ int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, i
   // for n <= 6 (79x80x80 n=6 475 against 565)
   p->items_bpc = (n >= 7) ? 16 : 32;
   p->stream_planes = 0;
+  // The one-item kernel with 16-byte OWN loads for periodic steps: 60^3 n=7
+  // 249-253 us against 274 us for the streaming kernel's column walk, 64^3
+  // n=8 410 against 449, 128^3 n=4 413 against 477.
+  p->flow = 2;
 
   // Message sizes, gpu/v-26-messages/Faces.cc:44-47, 64-66.
   const long nn = (long)n * n;
@@ -729,13 +783,12 @@ int faces_plan_create(faces_plan **out, int rank, int lx, int ly, int lz, int mx
     return hip_fail(e, "hipMalloc(message arena)");
   }
   // Receives from a missing neighbour leave zeros (gpu/DArray.h:81).  The
-  // clear runs on a stream of its own and only that stream is waited for, so
-  // creating a plan never waits for other work on the device.
+  // clear runs on the library's own setup stream and only that stream is
+  // waited for, so creating a plan never waits for other work on the device.
   hipStream_t zs = nullptr;
-  e = hipStreamCreateWithFlags(&zs, hipStreamNonBlocking);
+  e = setup_stream(&zs);
   if (e == hipSuccess) e = hipMemsetAsync(p->arena, 0, sizeof(double) * (size_t)off, zs);
   if (e == hipSuccess) e = hipStreamSynchronize(zs);
-  if (zs) (void)hipStreamDestroy(zs);
   if (e != hipSuccess) {
     (void)hipFree(p->arena);
     delete p;
@@ -818,7 +871,8 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       p->seg_len = value;
       return FACES_OK;
     case FACES_OPT_XCD:
-      p->xcd_remap = value ? 1 : 0;
+      if (value < 0 || value > 2) return fail(FACES_ERR_ARG, "XCD order %d outside [0,2]", value);
+      p->xcd_remap = value;
       return FACES_OK;
     case FACES_OPT_LDS_BLOCK:
       if (value < 64 || value > 512 || value % 64)
@@ -826,12 +880,12 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       p->lds_block = value;
       return FACES_OK;
     case FACES_OPT_STREAM_BPC:
-      if (value < -8 || value == -1 || value > 32)
-        return fail(FACES_ERR_ARG, "blocks per CU %d outside [-8,-2] and [0,32]", value);
+      if (value < -4 || value == -1 || value > 32)
+        return fail(FACES_ERR_ARG, "blocks per CU %d outside [-4,-2] and [0,32]", value);
       p->stream_bpc = value;
       return FACES_OK;
     case FACES_OPT_STREAM_WALK:
-      if (value < 0 || value > 6) return fail(FACES_ERR_ARG, "walk %d outside [0,6]", value);
+      if (value < 0 || value > 3) return fail(FACES_ERR_ARG, "walk %d outside [0,3]", value);
       p->stream_walk = value;
       return FACES_OK;
     case FACES_OPT_STREAM_ROWS:
@@ -840,6 +894,10 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       return FACES_OK;
     case FACES_OPT_STREAM_PLANES:
       p->stream_planes = value ? 1 : 0;
+      return FACES_OK;
+    case FACES_OPT_FLOW:
+      if (value < 0 || value > 2) return fail(FACES_ERR_ARG, "one-item kernel mode %d outside [0,2]", value);
+      p->flow = value;
       return FACES_OK;
     default: return fail(FACES_ERR_ARG, "unknown option %d", option);
   }
@@ -859,6 +917,7 @@ int faces_plan_get_option(const faces_plan *p, int option, int *value)
     case FACES_OPT_STREAM_ROWS: *value = p->stream_rows; break;
     case FACES_OPT_STREAM_PLANES: *value = stream_big(p) ? 1 : 0; break;
     case FACES_OPT_ACTIVE_KERNEL: *value = active_kernel(p); break;
+    case FACES_OPT_FLOW: *value = p->flow; break;
     default: return fail(FACES_ERR_ARG, "unknown option %d", option);
   }
   return FACES_OK;
@@ -870,10 +929,22 @@ int faces_plan_tune(faces_plan *p, double *u, int reps, void *stream, double *ou
   if (st) return st;
   if (reps < 1 || reps > 1000) return fail(FACES_ERR_ARG, "reps %d outside [1,1000]", reps);
   struct Cand {
-    int variant, walk, rows, seg, bpc;
+    int variant, walk, rows, seg, bpc, flow, xcd;
   };
   std::vector<Cand> cands;
   const int seg0 = p->seg_len;
+  if (stream_ok(p) && !stream_big(p)) {
+    // the one-item kernel: the three longest item lengths, both loads, both
+    // XCD orders
+    std::vector<int> segs;
+    for (int sl = 32; sl >= 1 && segs.size() < 3; sl--) {
+      p->seg_len = sl;
+      if (flow_seg(p) == sl) segs.push_back(sl);
+    }
+    for (int fl : {2, 1})
+      for (int sl : segs)
+        for (int x : {2, 1}) cands.push_back({FACES_VARIANT_STREAM, p->stream_walk, p->stream_rows, sl, p->stream_bpc, fl, x});
+  }
   if (stream_ok(p)) {
     // the three longest compiled segment lengths that divide mx
     std::vector<int> segs;
@@ -886,19 +957,23 @@ int faces_plan_tune(faces_plan *p, double *u, int reps, void *stream, double *ou
     const int wr[5][2] = {{1, 1}, {2, 0}, {2, 10}, {3, 2}, {3, 4}};
     for (const auto &w : wr)
       for (int sl : segs)
-        for (int bpc : {1, 2, 3, 4, -2, -3}) cands.push_back({FACES_VARIANT_STREAM, w[0], w[1], sl, bpc});
+        for (int bpc : {1, 2, 3, 4, -2, -3}) cands.push_back({FACES_VARIANT_STREAM, w[0], w[1], sl, bpc, 0, 1});
   }
   p->seg_len = seg0;
   if (items_ok(p))
-    for (int bpc : {2, 4, 8, 16, 32, 64}) cands.push_back({FACES_VARIANT_ITEMS, p->stream_walk, p->stream_rows, 0, bpc});
+    for (int bpc : {2, 4, 8, 16, 32, 64})
+      cands.push_back({FACES_VARIANT_ITEMS, p->stream_walk, p->stream_rows, 0, bpc, 0, 1});
   if (cands.empty()) return fail(FACES_ERR_STATE, "no tunable kernel for this grid");
 
   const int v0 = p->variant, w0 = p->stream_walk, r0 = p->stream_rows, b0 = p->stream_bpc, i0 = p->items_bpc;
+  const int f0 = p->flow, x0 = p->xcd_remap;
   auto apply = [&](const Cand &c) {
     p->variant = c.variant;
     p->stream_walk = c.walk;
     p->stream_rows = c.rows;
     p->seg_len = c.seg;
+    p->flow = c.flow;
+    p->xcd_remap = c.xcd;
     if (c.variant == FACES_VARIANT_ITEMS) p->items_bpc = c.bpc;
     else p->stream_bpc = c.bpc;
   };
@@ -935,14 +1010,18 @@ int faces_plan_tune(faces_plan *p, double *u, int reps, void *stream, double *ou
     p->seg_len = seg0;
     p->stream_bpc = b0;
     p->items_bpc = i0;
+    p->flow = f0;
+    p->xcd_remap = x0;
     return st;
   }
   apply(bestc);
   if (out) {
     out[0] = bestc.variant;
-    out[1] = bestc.walk;
-    out[2] = bestc.variant == FACES_VARIANT_ITEMS ? batch_seg(p->n, p->mx, 64) : stream_seg(p);
-    out[3] = bestc.bpc;
+    out[1] = bestc.flow ? -bestc.flow : bestc.walk;
+    out[2] = bestc.variant == FACES_VARIANT_ITEMS ? batch_seg(p->n, p->mx, 64)
+             : bestc.flow                         ? flow_seg(p)
+                                                  : stream_seg(p);
+    out[3] = bestc.flow ? bestc.xcd : bestc.bpc;
     out[4] = best;
   }
   return FACES_OK;

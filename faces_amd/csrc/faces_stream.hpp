@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
   const int units = (I.walk == 1) ? I.my * I.mz
                     : (I.walk == 2) ? I.mz * I.nseg * (I.my / L)
                     : (I.walk == 3) ? I.mz * (I.my / R)
-                                    : I.nitems;  // walk 0, walk 4
+                                    : I.nitems;
   int start, cnt, k, nbx;
   if (I.xcd_remap) {
     const int xcd = blockIdx.x & 7;
@@ -275,16 +275,6 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(Geom G, StreamItems I
     // walk 2: c = (jz * (my / L) + piece) * nseg + segment; its rows start at
     // jz * my + piece * L = (c / nseg) * L
     if (I.walk == 2) return ((c / I.nseg) * L + ss) * I.nseg + (c % I.nseg);
-    if (I.walk == 4) {
-      // bands of B = I.rows planes; within a band, rows in y, the band's
-      // planes of one y innermost: the z and y partners of an item come
-      // nseg and B*nseg items before it
-      const int B = I.rows, seg = c % I.nseg, r = c / I.nseg;
-      const int b = r / (B * I.my), rr = r - b * (B * I.my);
-      const int d = min(B, I.mz - b * B);
-      const int jy = rr / d, jz = b * B + (rr - (rr / d) * d);
-      return (jz * I.my + jy) * I.nseg + seg;
-    }
     return c;
   };
   double *const ring = lds + SH::P;  // saved seam columns, one per row of the group

@@ -125,7 +125,11 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *   FACES_OPT_VARIANT        FACES_VARIANT_*
  *   FACES_OPT_DIRECT_BLOCK   threads per block of the DIRECT kernel
  *   FACES_OPT_SEG            elements per work item (0 = default)
- *   FACES_OPT_XCD            1 = contiguous work ranges per XCD
+ *   FACES_OPT_XCD            XCD work order: 2 (default) = each XCD takes an
+ *                            eighth of every plane's items (the one-item
+ *                            kernel; the others read it as 1), 1 =
+ *                            contiguous eighths of the work list, 0 = one
+ *                            global order
  *   FACES_OPT_LDS_BLOCK      threads per block of the LDS kernel
  *   FACES_OPT_STREAM_BPC     STREAM kernel: persistent blocks per CU (1-32);
  *                            0 = one work unit per block; -2, -3, -4 = one
@@ -148,7 +152,13 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *   FACES_OPT_STREAM_PLANES  STREAM kernel addressing: 1 = through one
  *                            descriptor per jz plane (what fields of 4 GB and
  *                            more use; set, it applies to any field); get:
- *                            whether the plan addresses per plane */
+ *                            whether the plan addresses per plane
+ *   FACES_OPT_FLOW           periodic steps with compute = 1 (fields below
+ *                            4 GB): 2 (default) = the one-item kernel, one
+ *                            work item per block in memory order, OWN runs
+ *                            loaded 16 bytes per lane (8 when the field is not
+ *                            16-byte aligned); 1 = the same with 8-byte lanes;
+ *                            0 = the STREAM walks above */
 #define FACES_OPT_VARIANT 0
 #define FACES_OPT_DIRECT_BLOCK 1
 #define FACES_OPT_SEG 2
@@ -159,18 +169,21 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
 #define FACES_OPT_ACTIVE_KERNEL 7
 #define FACES_OPT_STREAM_ROWS 8
 #define FACES_OPT_STREAM_PLANES 9
+#define FACES_OPT_FLOW 10
 int faces_plan_set_option(faces_plan *plan, int option, int value);
 int faces_plan_get_option(const faces_plan *plan, int option, int *value);
 
 /* Autotuner of the sum kernel (no counterpart in the reference, whose launch
  * shape is fixed at build time, gpu/gpu.h:73-78): times the streaming kernel
- * over walks, segment lengths and blocks per CU, and the work-list kernel over
+ * over walks, segment lengths and blocks per CU, the one-item kernel over
+ * item lengths, load widths and XCD orders, and the work-list kernel over
  * blocks per CU, on this plan's grid, and keeps the fastest.  Every trial
  * shares `u` 1 + reps times (compute = 1): the field changes as after that
  * many shares, so tune on a scratch copy.  Synchronises `stream`.  out (may
- * be NULL), 5 doubles: variant, walk, elements per item, blocks per CU (as
- * FACES_OPT_STREAM_BPC: negative = one work unit per block, at most that many
- * resident per CU), µs per share of the choice. */
+ * be NULL), 5 doubles: variant, walk (the one-item kernel: -FACES_OPT_FLOW),
+ * elements per item, blocks per CU (as FACES_OPT_STREAM_BPC: negative = one
+ * work unit per block, at most that many resident per CU; the one-item
+ * kernel: its FACES_OPT_XCD order), µs per share of the choice. */
 int faces_plan_tune(faces_plan *plan, double *u, int reps, void *stream, double *out);
 
 /* Neighbour rank of message slot 0..25, or -1 for none (MPI_PROC_NULL) --

@@ -112,6 +112,7 @@ def test_stream_walks_bitexact(torch_cuda, walk, rows, seg, bpc, xcd, shape):
     d = to_dev(torch_cuda, u)
     f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
     f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+    f.set_option(fa.OPT_FLOW, 0)  # the streaming kernel's walks, not the one-item kernel
     f.set_option(fa.OPT_STREAM_WALK, walk)
     f.set_option(fa.OPT_STREAM_ROWS, rows)
     f.set_option(fa.OPT_SEG, seg)
@@ -123,6 +124,77 @@ def test_stream_walks_bitexact(torch_cuda, walk, rows, seg, bpc, xcd, shape):
     got = to_host(torch_cuda, d)
     f.close()
     assert_same(got, run_oracle(u, shape, nshare=2), shape)
+
+
+# --------------------------------- the one-item kernel (the default periodic step)
+
+FLOW_SHAPES = [(7, 12, 6, 3), (7, 10, 3, 2), (8, 4, 3, 5), (5, 10, 2, 3), (5, 6, 1, 2), (4, 16, 4, 1),
+               (6, 5, 2, 2), (2, 32, 2, 1), (3, 16, 3, 2), (9, 3, 2, 2), (10, 2, 3, 1), (11, 2, 1, 2),
+               (12, 1, 2, 2), (7, 1, 1, 1), (7, 15, 9, 2)]
+
+
+@pytest.mark.parametrize("flow", [1, 2])
+@pytest.mark.parametrize("xcd", [0, 1, 2])
+@pytest.mark.parametrize("shape", FLOW_SHAPES)
+@pytest.mark.parametrize("seg", [0, 1, 3, 5])
+def test_flow_bitexact(torch_cuda, flow, xcd, shape, seg):
+    """The one-item kernel: 8- and 16-byte loads, each XCD order, item
+    lengths whose first element starts on an even or an odd double (odd n,
+    odd segment starts), planes with fewer items than XCDs; two shares of
+    a random field, bit-exact against the oracle."""
+    n, mx, my, mz = shape
+    print("case flow", flow, xcd, shape, seg, flush=True)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=313)
+    d = to_dev(torch_cuda, u)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+    f.set_option(fa.OPT_FLOW, flow)
+    f.set_option(fa.OPT_XCD, xcd)
+    f.set_option(fa.OPT_SEG, seg)
+    assert f.get_option(fa.OPT_ACTIVE_KERNEL) == fa.VARIANT_STREAM
+    for _ in range(2):
+        f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    assert_same(got, run_oracle(u, shape, nshare=2), shape)
+
+
+@pytest.mark.parametrize("flow", [1, 2])
+@pytest.mark.parametrize("shape,extra", [((7, 6, 5, 4), 1000), ((7, 5, 3, 3), 1), ((6, 4, 3, 2), 3),
+                                         ((5, 10, 2, 3), 512), ((8, 3, 3, 2), 7)])
+def test_flow_padded_planes(torch_cuda, flow, shape, extra):
+    """Padded plane strides, odd ones included (planes then start on odd
+    doubles, so the 16-byte loads change phase from plane to plane)."""
+    n, mx, my, mz = shape
+    ps = n ** 3 * mx * my + extra
+    print("case flow padded", flow, shape, ps, flush=True)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=17, ps=ps)
+    d = to_dev(torch_cuda, u)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC, plane_stride=ps)
+    f.set_option(fa.OPT_FLOW, flow)
+    for _ in range(3):
+        f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    assert np.array_equal(got.view(np.uint64), run_oracle(u, shape, ps=ps, nshare=3).view(np.uint64))
+
+
+def test_flow_misaligned_field(torch_cuda):
+    """A field that starts 8 bytes into its allocation: the 16-byte loads
+    are not taken (the kernel falls back to 8-byte lanes), results exact."""
+    shape = (7, 10, 4, 3)
+    n, mx, my, mz = shape
+    print("case flow misaligned", shape, flush=True)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=23)
+    buf = torch_cuda.zeros(u.size + 2, dtype=torch_cuda.float64, device="cuda")
+    buf[1:1 + u.size] = torch_cuda.from_numpy(u).to("cuda")
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    assert f.get_option(fa.OPT_FLOW) == 2
+    f.share(int(buf.data_ptr()) + 8)
+    got = to_host(torch_cuda, buf)
+    f.close()
+    assert got[0] == 0.0 and got[-1] == 0.0
+    assert_same(got[1:1 + u.size].copy(), run_oracle(u, shape), shape)
 
 
 def test_periodic_lds_padded_and_repeated(torch_cuda):
@@ -424,6 +496,9 @@ def test_invalid_arguments(torch_cuda):
     for bad in (-1, -5, 33):
         with pytest.raises(fa.FacesError):
             f.set_option(fa.OPT_STREAM_BPC, bad)
+    for opt, bad in ((fa.OPT_FLOW, 3), (fa.OPT_FLOW, -1), (fa.OPT_XCD, 3), (fa.OPT_STREAM_WALK, 4)):
+        with pytest.raises(fa.FacesError):
+            f.set_option(opt, bad)
     f.close()
 
 
@@ -471,6 +546,7 @@ def test_stream_long_segments_bitexact(torch_cuda, shape, seg, walk):
     d = to_dev(torch_cuda, u)
     f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
     f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+    f.set_option(fa.OPT_FLOW, 0)
     f.set_option(fa.OPT_SEG, seg)
     f.set_option(fa.OPT_STREAM_WALK, walk)
     f.set_option(fa.OPT_STREAM_ROWS, 2)
