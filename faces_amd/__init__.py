@@ -20,6 +20,7 @@ import re
 
 __all__ = [
     "FacesError", "Faces", "Batch", "Server", "fill", "field_doubles", "stream_ceiling", "lib",
+    "share_noalloc",
     "LIB_PATH", "OPEN", "PERIODIC", "PATH_AUTO", "PATH_FUSED", "PATH_STAGED",
     "PATH_STAGED_DIRECT", "PATH_SIX", "FILL_INT", "FILL_RAND", "FILL_REF", "header_symbols",
 ]
@@ -78,6 +79,8 @@ _sig("faces_plan_create", _c_int, ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _
      _c_int, _c_int, _c_int, _c_int, _c_long)
 _sig("faces_plan_destroy", _c_int, _vp)
 _sig("faces_share", _c_int, _vp, _vp, _c_int, _vp)
+_sig("faces_share_noalloc", _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+     _c_int, _c_long, _c_int, _vp)
 _sig("faces_plan_set_path", _c_int, _vp, _c_int)
 _sig("faces_plan_set_tuning", _c_int, _vp, _c_int, _c_int, _c_int)
 _sig("faces_plan_set_variant", _c_int, _vp, _c_int, _c_int)
@@ -151,6 +154,17 @@ def stream_ceiling(u, n=None, reps=20, stream=None) -> float:
     _check(_lib.faces_stream_ceiling(_addr(u, n), n, reps, _stream(stream), ctypes.byref(out)),
            "faces_stream_ceiling")
     return out.value
+
+
+def share_noalloc(u, rank, lx, ly, lz, mx, my, mz, n, boundary=PERIODIC, plane_stride=0,
+                  compute=True, stream=None) -> None:
+    """The staged step of one rank without a plan or any allocation
+    (faces_share_noalloc): its messages live in the library's per-device
+    scratch array, shared by every call on that device."""
+    need = field_doubles(n, mx, my, mz, plane_stride)
+    _check(_lib.faces_share_noalloc(_addr(u, need if need > 0 else None), rank, lx, ly, lz, mx, my, mz, n,
+                                    boundary, plane_stride, int(bool(compute)), _stream(stream)),
+           "faces_share_noalloc")
 
 
 def fill(u, n, mx, my, mz, kind=FILL_INT, seed=42, plane_stride=0, stream=None) -> None:

@@ -22,7 +22,7 @@ INCLUDE = os.path.join(ROOT, "include")
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["faces_plan.hip", "faces_batch.hip", "faces_serve.cpp"]
+HIP_SOURCES = ["faces_plan.hip", "faces_batch.hip", "faces_serve.cpp", "faces_noalloc.hip"]
 
 COMMON_FLAGS = [
     "-std=c++17",

@@ -91,6 +91,23 @@ int faces_plan_destroy(faces_plan *plan);
  * Allocates nothing; may be captured in a HIP graph. */
 int faces_share(faces_plan *plan, double *u, int compute, void *stream);
 
+/* The staged step (faces_share on FACES_PATH_STAGED_DIRECT: pack into the
+ * receive slots, interior sums, surface sums) of rank `rank` of an
+ * lx*ly*lz rank grid, without a plan and without allocating: its receive
+ * messages live in a device array of FACES_NOALLOC_SCRATCH_BYTES that the
+ * library holds on each GPU it runs on until the process ends.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The rank's 26 receive messages
+ * (Faces.cc:44-47,64-66, each padded to 512 bytes) must fit the array
+ * (16 MiB holds a 60^3 n=7 rank's 8.5 MB), else FACES_ERR_ARG.  Every call
+ * on one GPU shares that GPU's array and keeps the last call's contents:
+ * calls that overlap on one GPU (other streams, other threads) race.
+ * Stream-ordered like faces_share. */
+#define FACES_NOALLOC_SCRATCH_BYTES (16L * 1024 * 1024)
+int faces_share_noalloc(double *u, int rank, int lx, int ly, int lz, int mx, int my, int mz, int n,
+                        int boundary, long plane_stride, int compute, void *stream);
+
 /* Selects the execution path (FACES_PATH_*). */
 int faces_plan_set_path(faces_plan *plan, int path);
 
