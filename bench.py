@@ -311,16 +311,24 @@ def cpu_baseline(n, mx, my, mz, kind, budget_s, periodic=True):
                       f"by the oracle's OpenMP pass sweep; {dt * 1e3:.1f} ms/share"}
 
 
-def load_traffic(name):
+def load_traffic(name, plan_opts):
+    """Measured HBM traffic of the step's kernel (profiles/traffic_<name>.json,
+    from rocprofv3 FETCH_SIZE/WRITE_SIZE runs), with the kernel and commit it
+    was measured on; None when the file is missing or was measured with other
+    launch options than the plan's."""
     p = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
             d = json.load(f)
-        return d.get("hbm_bytes_per_launch")
     except (OSError, ValueError):
         return None
+    want = d.get("plan", {})
+    if any(plan_opts.get(k) != v for k, v in want.items() if k in plan_opts):
+        return None
+    return {"bytes": d.get("hbm_bytes_per_launch"), "kernel": d.get("kernel"),
+            "commit": d.get("measured_at_commit"), "source": d.get("source")}
 
 
 def main():
@@ -355,6 +363,8 @@ def main():
         name, val = o.split("=")
         f.set_option(getattr(fa, "OPT_" + name.upper()), int(val))
     stream = torch.cuda.current_stream()
+    plan_opts = {"flow": f.get_option(fa.OPT_FLOW), "xcd": f.get_option(fa.OPT_XCD),
+                 "seg": f.get_option(fa.OPT_SEG)}
 
     for i in range(args.warmup):
         f.share(us[i % nbuf], stream=stream)
@@ -401,6 +411,7 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         args.steps = nrep * per_rep
+    traffic = load_traffic(args.config if periodic else args.config + "_open", plan_opts)
     per = [evs[i].elapsed_time(evs[i + 1]) / per_rep for i in range(nrep)]
     ms = sum(per) / len(per)
     gbs = b_step / (ms * 1e-3) / 1e9
@@ -447,7 +458,10 @@ def main():
         "wall_s": round(wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": load_traffic(args.config if periodic else args.config + "_open")},
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_kernel": traffic["kernel"] if traffic else None,
+                     "traffic_measured_at": traffic["commit"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None},
         "copy_reference_gbs": None if copy_gbs is None else round(copy_gbs, 1),
         "frac_of_copy": None if copy_gbs is None else round(gbs / copy_gbs, 4),
         "inplace_stream_gbs": None if stream_gbs is None else round(stream_gbs, 1),
