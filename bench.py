@@ -81,6 +81,8 @@ def parse():
                     help="batched leg launch shape SEG,BLOCKS_PER_CU instead of the tuner's")
     ap.add_argument("--requests", type=int, default=2000, help="requests in the served leg")
     ap.add_argument("--streams", type=int, default=2, help="launch slots of the served leg")
+    ap.add_argument("--serve-bpc", type=int, default=0,
+                    help="served leg: blocks per CU of batched launches (-1: one per item; 0: default)")
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -177,7 +179,7 @@ def batched_leg(torch, fa, ngrids, steps, warmup, tuning=None):
             "tuned": {"seg_len": tuned["seg_len"], "blocks_per_cu": tuned["blocks_per_cu"]}}
 
 
-def served_leg(torch, fa, nreq, args_nstreams=2):
+def served_leg(torch, fa, nreq, args_nstreams=2, bpc=0):
     from faces_amd.batch import Server
     arrivals, shapes = served_workload(nreq)
     fields = []
@@ -188,6 +190,8 @@ def served_leg(torch, fa, nreq, args_nstreams=2):
     torch.cuda.synchronize()
     cap = 200e9
     srv = Server(footprint_cap=cap, max_batch=256, nstreams=args_nstreams)
+    if bpc:
+        srv.set_tuning(bpc)
     # warm the server's launch path on a few requests first, one of them
     # large (its plain-step plan is created when it is first submitted)
     warm = [0, 1, 2] + [i for i, sh in enumerate(shapes) if sh == LARGE_SHAPE][:1]
@@ -222,7 +226,7 @@ def served_leg(torch, fa, nreq, args_nstreams=2):
             "req_per_s": round(st["req_per_s"], 1), "p50_us": round(st["p50_us"], 1),
             "p99_us": round(st["p99_us"], 1), "max_us": round(st["max_us"], 1),
             "launches": int(st["launches"]), "footprint_cap_bytes": cap,
-            "streams": args_nstreams,
+            "streams": args_nstreams, "blocks_per_cu": bpc or "default",
             "GBs": round(nbytes / st["wall_s"] / 1e9, 2),
             "capacity_req_per_s": round(cap_st["req_per_s"], 1),
             "per_request_launch_req_per_s": round(nreq / per_req_s, 1),
@@ -426,7 +430,7 @@ def main():
         cpu = cpu_baseline(n, mx, my, mz, kind, args.cpu_seconds, periodic)
 
     batched = None if args.no_batch else batched_leg(torch, fa, args.grids, 50, 5, args.batch_tuning)
-    served = None if args.no_serve else served_leg(torch, fa, args.requests, args.streams)
+    served = None if args.no_serve else served_leg(torch, fa, args.requests, args.streams, args.serve_bpc)
 
     per_sorted = sorted(per)
     rec = {

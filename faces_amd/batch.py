@@ -28,6 +28,7 @@ _SIGS = {
     "faces_server_run_trace": (_c_int, _vp, _c_int, _dp, _ip, _ip, _ip, _ip, _c_int, _c_int,
                                ctypes.POINTER(_vp), _dp, _dp),
     "faces_server_stats": (_c_int, _vp, _dp),
+    "faces_server_set_tuning": (_c_int, _vp, _c_int),
 }
 
 
@@ -146,6 +147,10 @@ class Server:
                                              stats), "faces_server_run_trace")
         return list(lat), {"wall_s": stats[0], "req_per_s": stats[1], "p50_us": stats[2],
                            "p99_us": stats[3], "max_us": stats[4], "launches": stats[5]}
+
+    def set_tuning(self, blocks_per_cu):
+        """Blocks per CU of batched launches (1-256), or -1: one block per item."""
+        _check(_fn("faces_server_set_tuning")(self._h, blocks_per_cu), "faces_server_set_tuning")
 
     def stats(self):
         s = (ctypes.c_double * 4)()

@@ -124,6 +124,81 @@ __global__ __launch_bounds__(kStreamThreads, 4) void k_batch(BatchLaunch D)
   }
 }
 
+// ---------------------------------------------------------------------------
+// the batched kernel, one item per block
+// ---------------------------------------------------------------------------
+// Every block takes one item of the work list; XCD x (block b runs on XCD
+// b % 8) takes the x-th contiguous eighth of the list, dispatched in order.
+// A grid's items are in memory order, so an item's seam, y and z partners
+// (one, nseg and my*nseg items back) were summed moments before on the same
+// XCD and their shared lines meet in its L2; with no loop, short-lived
+// blocks keep many items' loads in flight per CU.
+
+template <int N, int S>
+__device__ __forceinline__ void batch_item(const BatchGrid G0, int it, double *lds)
+{
+  BatchGrid G;
+  G.u = (double *)uni((long)G0.u);
+  G.ys = uni(G0.ys);
+  G.ps = uni(G0.ps);
+  G.field_bytes = (unsigned)uni((int)G0.field_bytes);
+  G.n = uni(G0.n);
+  G.mx = uni(G0.mx);
+  G.my = uni(G0.my);
+  G.mz = uni(G0.mz);
+  G.S = uni(G0.S);
+  G.nseg = uni(G0.nseg);
+  G.items = uni(G0.items);
+  G.first = uni(G0.first);
+  item_once<N, S>(G, uni(it), lds);
+}
+
+__global__ __launch_bounds__(kStreamThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_batch_flow(BatchLaunch D)
+{
+  extern __shared__ double lds[];
+  const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+  const int q8 = D.nunits >> 3, rem8 = D.nunits & 7;
+  if (k >= q8 + (xcd < rem8 ? 1 : 0)) return;
+  const int u = xcd * q8 + (xcd < rem8 ? xcd : rem8) + k;
+  // the grid holding item u: the number of grids whose first item is at or
+  // before u, counted 64 grids per step across the wave's lanes (grids are
+  // sorted by first item)
+  int g = 0;
+  if (D.grids) {
+    const int lane = threadIdx.x & 63;
+    int cnt = 0;
+    for (int base = 0; base < D.ngrids; base += 256) {
+      int f[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gi = base + r * 64 + lane;
+        f[r] = gi < D.ngrids ? D.firsts[gi] : 0x7fffffff;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) cnt += __popcll(__ballot(f[r] <= u));
+    }
+    g = uni(cnt - 1);
+  }
+  const BatchGrid G = D.grids ? D.grids[g] : D.g0;
+  switch (G.n * 64 + G.S) {
+#define FACES_BATCH_CASE(NN, SS) \
+  case NN * 64 + SS: batch_item<NN, SS>(G, u - G.first, lds); break;
+    FACES_BATCH_CASES(FACES_BATCH_CASE)
+#undef FACES_BATCH_CASE
+    default: break;  // excluded on the host
+  }
+}
+
+size_t batch_pack_desc(const std::vector<BatchGrid> &g, void *buf)
+{
+  BatchGrid *grids = static_cast<BatchGrid *>(buf);
+  std::copy(g.begin(), g.end(), grids);
+  int *firsts = reinterpret_cast<int *>(grids + g.size());
+  for (size_t i = 0; i < g.size(); i++) firsts[i] = g[i].first;
+  return (sizeof(BatchGrid) + sizeof(int)) * g.size();
+}
+
 bool batch_case_ok(int n, int S)
 {
   if (S < 1 || S > 63) return false;
@@ -157,16 +232,23 @@ int batch_seg(int n, int mx, int cap)
 
 int batch_blocks(int nunits, int blocks_per_cu)
 {
+  if (blocks_per_cu == kBatchOneItem) return kXcds * ((nunits + kXcds - 1) / kXcds);
   long nb = (long)device_cus() * blocks_per_cu;
   nb = std::min(nb, (long)nunits + kXcds - 1);
   nb = (nb + kXcds - 1) / kXcds * kXcds;
   return (int)std::max((long)kXcds, nb);
 }
 
-int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t st)
+int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t st, bool one_item)
 {
   if (D.nunits == 0) return FACES_OK;
   if (nblocks % kXcds) return fail(FACES_ERR_ARG, "batched launch of %d blocks", nblocks);
+  if (one_item) {
+    if (nblocks < D.nunits) return fail(FACES_ERR_ARG, "one-item launch of %d blocks for %d items", nblocks, D.nunits);
+    k_batch_flow<<<dim3(nblocks), dim3(kStreamThreads), lds_bytes, st>>>(D);
+    FACES_HIP(hipGetLastError());
+    return FACES_OK;
+  }
   k_batch<<<dim3(nblocks), dim3(kStreamThreads), lds_bytes, st>>>(D);
   FACES_HIP(hipGetLastError());
   return FACES_OK;
@@ -200,10 +282,12 @@ int launch_single(double *u, int n, int mx, int my, int mz, long plane_stride, i
   x.cost0 = 0.0;
   x.icost = 1.0;
   D.grids = nullptr;
+  D.firsts = nullptr;
   D.ngrids = 1;
   D.nunits = x.items;
   D.total = (double)x.items;
-  return launch_batch(D, batch_blocks(x.items, blocks_per_cu), batch_lds_bytes(n, x.S), st);
+  return launch_batch(D, batch_blocks(x.items, blocks_per_cu), batch_lds_bytes(n, x.S), st,
+                      blocks_per_cu == kBatchOneItem);
 }
 
 int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, double *total)
@@ -253,10 +337,10 @@ struct faces_batch {
   BatchGrid *d_grids = nullptr;
   double total = 0.0;
   int lds_bytes = 0, nunits = 0, nbatched = 0;
-  // tuning
-  // 32 blocks per CU: 256 mixed grids (n 4-8, 2-24 elements a side) 681 us
-  // against 721 us for 16, 772 us for 8
-  int seg_cap = 64, bpc = 32;
+  // tuning: one block per item (256 mixed grids, n 4-8, 2-24 elements a
+  // side: 626 us, traffic 1.02 x the fields; chunks of the list with 32
+  // blocks per CU 685 us, traffic 1.36 x)
+  int seg_cap = 64, bpc = kBatchOneItem;
   // captured graph of the batched launch
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -302,7 +386,9 @@ int upload(faces_batch *b)
     if (st) return st;
     // launches of the old descriptors may still run on any stream
     FACES_HIP(hipDeviceSynchronize());
-    FACES_HIP(hipMemcpy(b->d_grids, g.data(), sizeof(BatchGrid) * g.size(), hipMemcpyHostToDevice));
+    std::vector<char> host(batch_desc_bytes((int)g.size()));
+    const size_t bytes = batch_pack_desc(g, host.data());
+    FACES_HIP(hipMemcpy(b->d_grids, host.data(), bytes, hipMemcpyHostToDevice));
   }
   b->nbatched = (int)g.size();
   if (b->exec) {
@@ -321,6 +407,7 @@ BatchLaunch launch_desc(const faces_batch *b)
 {
   BatchLaunch D = {};
   D.grids = b->d_grids;
+  D.firsts = reinterpret_cast<const int *>(b->d_grids + b->nbatched);
   D.ngrids = b->nbatched;
   D.nunits = b->nunits;
   D.total = b->total;
@@ -337,7 +424,7 @@ int ensure_graph(faces_batch *b)
   }
   if (b->exec || b->nunits == 0) return FACES_OK;
   FACES_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-  int st = launch_batch(launch_desc(b), nblocks_of(b), b->lds_bytes, b->cap_stream);
+  int st = launch_batch(launch_desc(b), nblocks_of(b), b->lds_bytes, b->cap_stream, b->bpc == kBatchOneItem);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
   if (st) {
@@ -398,7 +485,7 @@ int faces_batch_create(faces_batch **out, int ngrids, const int *n, const int *m
     if (!st) b->plans.push_back(p);
   }
   hipError_t e = hipSuccess;
-  if (!st) e = hipMalloc(&b->d_grids, sizeof(BatchGrid) * ngrids);
+  if (!st) e = hipMalloc(&b->d_grids, batch_desc_bytes(ngrids));
   if (!st && e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
   if (!st && e == hipSuccess) e = hipDeviceSynchronize();
   if (!st && e != hipSuccess) st = hip_fail(e, "batch device state");
@@ -442,8 +529,8 @@ int faces_batch_set_tuning(faces_batch *b, int block, int seg_len, int blocks_pe
   if (block != 0 && block != kStreamThreads)
     return fail(FACES_ERR_ARG, "batched kernel blocks have %d threads", kStreamThreads);
   if (seg_len < 0 || seg_len > 64) return fail(FACES_ERR_ARG, "segment length %d outside [0,64]", seg_len);
-  if (blocks_per_cu < 0 || blocks_per_cu > 256)
-    return fail(FACES_ERR_ARG, "blocks per CU %d outside [0,256]", blocks_per_cu);
+  if (blocks_per_cu < kBatchOneItem || blocks_per_cu > 256)
+    return fail(FACES_ERR_ARG, "blocks per CU %d outside [-1,256]", blocks_per_cu);
   if (seg_len) b->seg_cap = seg_len;
   if (blocks_per_cu) b->bpc = blocks_per_cu;
   b->dirty = true;
@@ -507,7 +594,7 @@ int faces_batch_tune(faces_batch *b, int reps, void *stream, double *out)
   int best_seg = b->seg_cap, best_bpc = b->bpc;
   int rc = FACES_OK;
   for (int seg : {2, 4, 8, 16, 64}) {
-    for (int bpc : {2, 4, 8, 16, 32, 48, 64, 96, 128}) {
+    for (int bpc : {kBatchOneItem, 2, 4, 8, 16, 32, 48, 64, 96, 128}) {
       b->seg_cap = seg;
       b->bpc = bpc;
       b->dirty = true;
@@ -515,10 +602,10 @@ int faces_batch_tune(faces_batch *b, int reps, void *stream, double *out)
       if (rc) break;
       const BatchLaunch D = launch_desc(b);
       const int nb = nblocks_of(b);
-      rc = launch_batch(D, nb, b->lds_bytes, st);  // warm-up
+      rc = launch_batch(D, nb, b->lds_bytes, st, bpc == kBatchOneItem);  // warm-up
       for (int r = 0; r < reps && !rc; r++) {
         if (r == 0 && hipEventRecord(e0, st) != hipSuccess) rc = fail(FACES_ERR_HIP, "event");
-        if (!rc) rc = launch_batch(D, nb, b->lds_bytes, st);
+        if (!rc) rc = launch_batch(D, nb, b->lds_bytes, st, bpc == kBatchOneItem);
       }
       if (rc) break;
       float ms = 0;

@@ -17,11 +17,20 @@ namespace faces {
 constexpr int kXcds = 8;
 struct BatchLaunch {
   const BatchGrid *grids;  // null: the single grid g0 (passed by value)
+  const int *firsts;       // grids[i].first as a compact array (follows the grids in their buffer)
   int ngrids;
   int nunits;
   double total;            // estimated cost of the whole list
   BatchGrid g0;
 };
+
+// Device (and pinned staging) buffer of a launch's descriptors for up to
+// `cap` grids: the BatchGrid array, then the grids' first items as ints.
+inline size_t batch_desc_bytes(int cap) { return (sizeof(BatchGrid) + sizeof(int)) * (size_t)cap; }
+
+// Writes the descriptors of g into a buffer of that layout (grids, then
+// their first items right after the last grid); returns the bytes written.
+size_t batch_pack_desc(const std::vector<BatchGrid> &g, void *buf);
 
 // Elements per item for a grid (largest S <= cap dividing mx with a kernel
 // specialisation), 0 if none.
@@ -33,11 +42,16 @@ int batch_lds_bytes(int n, int S);
 // estimated cost of the launch.
 int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, double *total);
 
-// Persistent blocks for a launch: blocks_per_cu per CU, a multiple of kXcds.
+// blocks_per_cu value that selects the one-item-per-block kernel.
+constexpr int kBatchOneItem = -1;
+
+// Blocks of a launch: blocks_per_cu per CU (a chunk of the list each), or,
+// for kBatchOneItem, one per item; a multiple of kXcds.
 int batch_blocks(int nunits, int blocks_per_cu);
 
-// Enqueues the batched kernel (nblocks blocks, a multiple of kXcds, a chunk each).
-int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t st);
+// Enqueues the batched kernel: nblocks blocks (a multiple of kXcds) taking a
+// chunk of the list each, or (one_item) one item each.
+int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t st, bool one_item = false);
 
 // One periodic grid (n <= 12, field below 4 GB) through the batched kernel:
 // element segments of up to seg_cap elements, the row's last one possibly

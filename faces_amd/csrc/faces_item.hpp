@@ -217,4 +217,36 @@ __device__ __forceinline__ void stream_run(const BatchGrid &G, const Order &item
   __syncthreads();  // LDS is reused by the block's next run
 }
 
+// One item, loaded, summed and stored by the calling block with nothing
+// else in flight (the batched step's one-item-per-block form): the slots that
+// hold no footprint location load nothing.
+template <int N, int S>
+__device__ __forceinline__ void item_once(const BatchGrid &G, int it, double *lds)
+{
+  using SH = StreamShape<N, S>;
+  constexpr int J = SH::J;
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  const int tid = threadIdx.x;
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(G.u, 0, (int)G.field_bytes, 0x00020000);
+  unsigned ld[J], st[J];
+  const int ecnt = item_offsets<N, S>(G, it, ld, st);
+  double v[J];
+#pragma unroll
+  for (int j = 0; j < J; j++) {
+    const int q = j * kStreamThreads + tid;
+    const bool used = (j < SH::JO) ? q < SH::cOWN : q - SH::JO * kStreamThreads < SH::cPART;
+    v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, used ? ld[j] : kOobBase, 0, 0));
+  }
+#pragma unroll
+  for (int j = 0; j < J; j++) lds[j * kStreamThreads + tid] = v[j];
+  __syncthreads();
+  item_sums<N, S>(lds, ecnt);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < J; j++)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, lds[j * kStreamThreads + tid]), rsrc,
+                                          st[j], 0, 0);
+}
+
 }  // namespace faces

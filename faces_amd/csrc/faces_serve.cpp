@@ -48,12 +48,11 @@ using Clock = std::chrono::steady_clock;
 // Requests at or above this footprint get a launch of their own.
 constexpr double kLargeBytes = 64.0 * 1024 * 1024;
 
-// Blocks per CU of a batched launch (each takes one contiguous chunk of the
-// launch's work list), with two launches in flight: 2 (Poisson 20k req/s:
-// p50 33 us, p99 250 us; 2000 requests queued at once: 98k req/s), against
-// 1 (p99 255 us, 93k req/s), 4 (p99 280-300 us, 94k req/s), 8 (87-88k) and
-// 32 (79k, the batched form's default).
-constexpr int kServeBlocksPerCu = 2;
+// Default launch shape of a batched launch, with two launches in flight: one
+// block per work item (Poisson 20k req/s: p50 32 us, p99 203 us), against
+// chunks of the launch's work list with 1, 2 or 4 blocks per CU (p50 39, 35,
+// 42 us; p99 220, 219, 228 us).
+constexpr int kServeBlocksPerCu = kBatchOneItem;
 
 double now_us(Clock::time_point t0) { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
 
@@ -90,6 +89,7 @@ struct Slot {
 struct faces_server {
   double cap;
   int max_batch;
+  int bpc = kServeBlocksPerCu;  // launch shape of batched launches (faces_server_set_tuning)
   int device = 0;
   std::vector<Slot> slots;
   Clock::time_point t0;
@@ -155,15 +155,15 @@ int faces_server::launch(Slot &s, std::vector<Request *> &batch)
     double total = 0.0;
     int st = plan_grids(g, 64, &lds, &nunits, &total);
     if (st) return st;
-    std::copy(g.begin(), g.end(), s.h_grids);
-    FACES_HIP(hipMemcpyAsync(s.d_grids, s.h_grids, sizeof(BatchGrid) * g.size(), hipMemcpyHostToDevice,
-                             s.stream));
+    const size_t bytes = batch_pack_desc(g, s.h_grids);
+    FACES_HIP(hipMemcpyAsync(s.d_grids, s.h_grids, bytes, hipMemcpyHostToDevice, s.stream));
     BatchLaunch D = {};
     D.grids = s.d_grids;
+    D.firsts = reinterpret_cast<const int *>(s.d_grids + g.size());
     D.ngrids = (int)g.size();
     D.nunits = nunits;
     D.total = total;
-    st = launch_batch(D, batch_blocks(nunits, kServeBlocksPerCu), lds, s.stream);
+    st = launch_batch(D, batch_blocks(nunits, bpc), lds, s.stream, bpc == kBatchOneItem);
     if (st) return st;
   }
   for (Request *r : other) {
@@ -290,8 +290,8 @@ int faces_server_create(faces_server **out, double footprint_cap, int max_batch,
   for (Slot &sl : s->slots) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.event, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipMalloc(&sl.d_grids, sizeof(BatchGrid) * max_batch);
-    if (e == hipSuccess) e = hipHostMalloc(&sl.h_grids, sizeof(BatchGrid) * max_batch, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&sl.d_grids, batch_desc_bytes(max_batch));
+    if (e == hipSuccess) e = hipHostMalloc(&sl.h_grids, batch_desc_bytes(max_batch), hipHostMallocDefault);
   }
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
@@ -466,6 +466,16 @@ int faces_server_run_trace(faces_server *s, int nreq, const double *arrival_us, 
     std::lock_guard<std::mutex> lk(s->mu);
     stats[5] = (double)(s->n_launches - launches0);
   }
+  return FACES_OK;
+}
+
+int faces_server_set_tuning(faces_server *s, int blocks_per_cu)
+{
+  if (!s) return fail(FACES_ERR_ARG, "null server");
+  if (blocks_per_cu < kBatchOneItem || blocks_per_cu == 0 || blocks_per_cu > 256)
+    return fail(FACES_ERR_ARG, "blocks per CU %d outside {-1} and [1,256]", blocks_per_cu);
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->bpc = blocks_per_cu;
   return FACES_OK;
 }
 

@@ -263,8 +263,10 @@ int faces_batch_share(faces_batch *batch, int compute, void *stream);
 int faces_batch_share_per_grid(faces_batch *batch, int compute, void *stream);
 
 /* Launch shape: threads per block (0 or 256), longest work item in elements
- * (1-64), blocks per CU (1-256; each block takes one contiguous chunk of the
- * work list of equal estimated cost); 0 keeps the current value. */
+ * (1-64), blocks per CU (1-256: each block takes one contiguous chunk of the
+ * work list of equal estimated cost; -1: one block per work item, the
+ * items of XCD x's eighth of the list dispatched in order); 0 keeps the
+ * current value. */
 int faces_batch_set_tuning(faces_batch *batch, int block, int seg_len, int blocks_per_cu);
 
 /* Times every candidate launch shape on the bound fields (reps replays each,
@@ -305,6 +307,11 @@ int faces_server_run_trace(faces_server *srv, int nreq, const double *arrival_us
 
 /* stats[0..3] = admitted, completed, launches, peak admitted bytes. */
 int faces_server_stats(const faces_server *srv, double *stats);
+
+/* Launch shape of the server's batched launches: blocks per CU (1-256, a
+ * chunk of the launch's work list each), or -1 (default) for one block per
+ * work item.  Applies to launches made after the call. */
+int faces_server_set_tuning(faces_server *srv, int blocks_per_cu);
 
 #ifdef __cplusplus
 }

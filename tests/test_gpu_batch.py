@@ -81,17 +81,23 @@ EDGE_SHAPES = edge_shapes()
 
 
 @pytest.mark.parametrize("kind", [fa.FILL_INT, fa.FILL_RAND])
-def test_batch_edge_shapes_bitexact(torch_cuda, kind):
-    print("case batch edge shapes", kind, flush=True)
+@pytest.mark.parametrize("bpc", [0, -1, 32])
+def test_batch_edge_shapes_bitexact(torch_cuda, kind, bpc):
+    """Every routine, ragged rows included, with the default launch, one
+    block per item, and chunks of the list."""
+    print("case batch edge shapes", kind, bpc, flush=True)
     fields = make_fields(torch_cuda, EDGE_SHAPES, kind=kind, seed0=50)
     b = Batch(EDGE_SHAPES)
     b.set_fields(fields)
+    if bpc:
+        b.set_tuning(256, 0, bpc)
     b.share()
     assert_all_same(host(torch_cuda, fields), oracle_steps(EDGE_SHAPES, kind, 50, 1), EDGE_SHAPES)
     b.close()
 
 
-@pytest.mark.parametrize("seg,bpc", [(1, 1), (2, 2), (3, 8), (4, 1), (8, 6), (64, 32)])
+@pytest.mark.parametrize("seg,bpc", [(1, 1), (2, 2), (3, 8), (4, 1), (8, 6), (64, 32), (1, -1), (4, -1),
+                                     (64, -1)])
 def test_batch_tuning_bitexact(torch_cuda, seg, bpc):
     print("case batch tuning", seg, bpc, flush=True)
     shapes = mixed_shapes(40, seed=7, ns=(2, 3, 4, 5, 6, 7, 8), lo=1, hi=9)
@@ -160,7 +166,8 @@ def test_batch_tuner(torch_cuda):
     b = Batch(shapes)
     b.set_fields(fields)
     res = b.tune(reps=2)
-    assert res["seg_len"] in (2, 4, 8, 16, 64) and res["blocks_per_cu"] in (1, 2, 3, 4, 6, 8)
+    assert res["seg_len"] in (2, 4, 8, 16, 64)
+    assert res["blocks_per_cu"] in (-1, 2, 4, 8, 16, 32, 48, 64, 96, 128)
     assert res["us_per_step"] > 0
     # the tuner stepped the fields; restore them and check one step
     for d, k in zip(fields, keep):
