@@ -35,13 +35,16 @@ struct FlowItems {
                          // 0: block b takes item b
 };
 
+// SKIP: the staged step's interior sums (faces_inner): nodes on the rank
+// surface -- those whose partner across a periodic wrap the item holds --
+// are left as they are, for the surface sums that add the received copies.
 // W16: the OWN runs are loaded 16 bytes per lane (the field must be 16-byte
 // aligned).  An element's run [b, b + (n-1)n^2) has even length; it takes
 // (n-1)n^2/2 + 1 pair slots, of which an even b uses the first (n-1)n^2/2 and
 // an odd b all, starting one double early and ending one late (the halves
 // outside the run are dropped on the way to LDS).  Stores stay 8 bytes per
 // lane: a pair may hold one location this item owns and one it must not write.
-template <int N, int S, bool W16>
+template <int N, int S, bool W16, bool SKIP = false>
 __global__ __launch_bounds__(kStreamThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_flow(Geom G, FlowItems I)
 {
@@ -208,8 +211,11 @@ void k_flow(Geom G, FlowItems I)
     }
 #pragma unroll
     for (int w = 0; w < 4; w++) np[i][w] = p[2 * w] | (p[2 * w + 1] << 16);
-    nflag[i] = (ix == 0 ? 1u : 0u) | (iy == 0 ? 2u : 0u) | (iz == 0 ? 4u : 0u) | 8u;
+    nflag[i] = (ix == 0 ? 1u : 0u) | (iy == 0 ? 2u : 0u) | (iz == 0 ? 4u : 0u) | 8u | (e == 0 ? 16u : 0u);
   }
+  // SKIP: axes on which this item holds the rank surface (its partners
+  // across it are the wrap partners: segment 0, row jy = 0, plane jz = 0)
+  const unsigned surf = SKIP ? ((sg == 0 ? 1u : 0u) | (jy == 0 ? 2u : 0u) | (jz == 0 ? 4u : 0u)) : 0u;
 
   if constexpr (W16) {
     typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -232,6 +238,9 @@ void k_flow(Geom G, FlowItems I)
   for (int i = 0; i < NN; i++) {
     const unsigned f = nflag[i];
     if (!(f & 8u)) continue;
+    // a node whose x pair (element 0 of the item only), y or z pair crosses
+    // the rank surface is the surface sums'
+    if (SKIP && (surf & ((f & 6u) | ((f & 17u) == 17u ? 1u : 0u)))) continue;
     const unsigned p0 = np[i][0] & 0xffff, p1 = np[i][0] >> 16;
     const unsigned p2 = np[i][1] & 0xffff, p3 = np[i][1] >> 16;
     const unsigned p4 = np[i][2] & 0xffff, p5 = np[i][2] >> 16;

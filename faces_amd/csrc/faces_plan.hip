@@ -340,7 +340,7 @@ bool stream_ok(const faces_plan *p)
 template <int N, int S>
 int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t st)
 {
-  if (p->flow && mode == 0 && !stream_big(p)) {
+  if (p->flow && mode <= 1 && !stream_big(p)) {
     // one item per block, items in memory order (faces_flow.hpp)
     FlowItems F;
     F.my = p->my;
@@ -359,11 +359,16 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t s
       nb = 8u * (unsigned)((F.nitems + 7) / 8);
     }
     const size_t lds = (size_t)StreamShape<N, S>::P * sizeof(double);
-    // 16-byte OWN loads need a 16-byte aligned field
-    if (p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0)
-      k_flow<N, S, true><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
-    else
-      k_flow<N, S, false><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
+    // 16-byte OWN loads need a 16-byte aligned field; mode 1 skips the rank
+    // surface (the staged step's interior sums)
+    const bool w16 = p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0;
+    if (mode == 1) {
+      if (w16) k_flow<N, S, true, true><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
+      else k_flow<N, S, false, true><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
+    } else {
+      if (w16) k_flow<N, S, true><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
+      else k_flow<N, S, false><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
+    }
     FACES_HIP(hipGetLastError());
     return FACES_OK;
   }
@@ -470,7 +475,7 @@ int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, hipStre
 int launch_stream(const faces_plan *p, double *u, int mode, hipStream_t st)
 {
   const Geom G = make_geom(p, u);
-  const int S = (p->flow && mode == 0 && !stream_big(p)) ? flow_seg(p) : stream_seg(p);
+  const int S = (p->flow && mode <= 1 && !stream_big(p)) ? flow_seg(p) : stream_seg(p);
   if (S == 0) return fail(FACES_ERR_ARG, "no streaming item length for n=%d mx=%d", p->n, p->mx);
   switch (p->n) {
     case 2: return launch_stream_n<2>(p, G, S, mode, st);

@@ -170,8 +170,9 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *                            descriptor per jz plane (what fields of 4 GB and
  *                            more use; set, it applies to any field); get:
  *                            whether the plan addresses per plane
- *   FACES_OPT_FLOW           periodic steps with compute = 1 (fields below
- *                            4 GB): 2 (default) = the one-item kernel, one
+ *   FACES_OPT_FLOW           periodic steps with compute = 1 and the staged
+ *                            step's interior sums (fields below 4 GB):
+ *                            2 (default) = the one-item kernel, one
  *                            work item per block in memory order, OWN runs
  *                            loaded 16 bytes per lane (8 when the field is not
  *                            16-byte aligned); 1 = the same with 8-byte lanes;
