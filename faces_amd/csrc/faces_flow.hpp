@@ -30,21 +30,27 @@ struct FlowItems {
   int nseg;              // mx / S
   int nitems;            // my*mz*nseg
   unsigned field_bytes;  // below kMaxBufferBytes: offsets are 32-bit
+  int nbr_mask;          // MODE 2: face neighbours of the rank, bit 2a (2a+1) for the low (high)
+                         // side of axis a (stood in for by self halos)
   int xcd_remap;         // 2: XCD x takes the x-th eighth of every plane's items, planes in
                          // order; 1: XCD x takes the x-th contiguous eighth of the list;
                          // 0: block b takes item b
 };
 
-// SKIP: the staged step's interior sums (faces_inner): nodes on the rank
-// surface -- those whose partner across a periodic wrap the item holds --
-// are left as they are, for the surface sums that add the received copies.
+// MODE 0: periodic (every node with its wrap partners).  MODE 1: the staged
+// step's interior sums (faces_inner): nodes on the rank surface -- those
+// whose partner across a periodic wrap the item holds -- are left as they
+// are, for the surface sums that add the received copies.  MODE 2: an open
+// rank: each side of the rank surface is a node of its own, which sums its
+// copies on that side, and those across the surface where the rank has a
+// neighbour there (stood in for by the self halo), x pairs, then y, then z.
 // W16: the OWN runs are loaded 16 bytes per lane (the field must be 16-byte
 // aligned).  An element's run [b, b + (n-1)n^2) has even length; it takes
 // (n-1)n^2/2 + 1 pair slots, of which an even b uses the first (n-1)n^2/2 and
 // an odd b all, starting one double early and ending one late (the halves
 // outside the run are dropped on the way to LDS).  Stores stay 8 bytes per
 // lane: a pair may hold one location this item owns and one it must not write.
-template <int N, int S, bool W16, bool SKIP = false>
+template <int N, int S, bool W16, int MODE = 0>
 __global__ __launch_bounds__(kStreamThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_flow(Geom G, FlowItems I)
 {
@@ -213,9 +219,9 @@ void k_flow(Geom G, FlowItems I)
     for (int w = 0; w < 4; w++) np[i][w] = p[2 * w] | (p[2 * w + 1] << 16);
     nflag[i] = (ix == 0 ? 1u : 0u) | (iy == 0 ? 2u : 0u) | (iz == 0 ? 4u : 0u) | 8u | (e == 0 ? 16u : 0u);
   }
-  // SKIP: axes on which this item holds the rank surface (its partners
+  // MODE 1, 2: axes on which this item holds the rank surface (its partners
   // across it are the wrap partners: segment 0, row jy = 0, plane jz = 0)
-  const unsigned surf = SKIP ? ((sg == 0 ? 1u : 0u) | (jy == 0 ? 2u : 0u) | (jz == 0 ? 4u : 0u)) : 0u;
+  const unsigned surf = MODE ? ((sg == 0 ? 1u : 0u) | (jy == 0 ? 2u : 0u) | (jz == 0 ? 4u : 0u)) : 0u;
 
   if constexpr (W16) {
     typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -240,7 +246,7 @@ void k_flow(Geom G, FlowItems I)
     if (!(f & 8u)) continue;
     // a node whose x pair (element 0 of the item only), y or z pair crosses
     // the rank surface is the surface sums'
-    if (SKIP && (surf & ((f & 6u) | ((f & 17u) == 17u ? 1u : 0u)))) continue;
+    if (MODE && (surf & ((f & 6u) | ((f & 17u) == 17u ? 1u : 0u)))) continue;
     const unsigned p0 = np[i][0] & 0xffff, p1 = np[i][0] >> 16;
     const unsigned p2 = np[i][1] & 0xffff, p3 = np[i][1] >> 16;
     const unsigned p4 = np[i][2] & 0xffff, p5 = np[i][2] >> 16;
@@ -268,6 +274,73 @@ void k_flow(Geom G, FlowItems I)
       if (cy) {
         lds[p6] = res;
         if (cx) lds[p7] = res;
+      }
+    }
+  }
+  // MODE 2: the rank-surface nodes of this item (block-uniform: only items on
+  // the surface have any); each side of the surface sums its copies, and
+  // those across the surface where the rank has a neighbour on that side
+  if constexpr (MODE == 2) {
+    if (surf) {
+#pragma unroll
+      for (int i = 0; i < NN; i++) {
+        const unsigned f = nflag[i];
+        if (!(f & 8u)) continue;
+        const unsigned wrap = surf & ((f & 6u) | ((f & 17u) == 17u ? 1u : 0u));
+        if (!wrap) continue;
+        // copy (sx | sy << 1 | sz << 2), side 1 = the partner across that axis;
+        // all eight are read (positions of copies the node does not have lie
+        // in the footprint too, and go unused)
+        const unsigned pp[8] = {np[i][0] & 0xffff, np[i][0] >> 16, np[i][1] & 0xffff, np[i][1] >> 16,
+                                np[i][2] & 0xffff, np[i][2] >> 16, np[i][3] & 0xffff, np[i][3] >> 16};
+        double c[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) c[m] = lds[pp[m]];
+        // pair[a][side]: that side of axis a sums both copies -- a pair inside
+        // the rank, or across the surface with a neighbour on that side
+        bool pair[3][2];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+          for (int sd = 0; sd < 2; sd++)
+            pair[a][sd] = (f & (1u << a)) && (!(wrap & (1u << a)) || (I.nbr_mask & (1 << (2 * a + sd))));
+        // the tree level by level for both sides of every axis
+        double A[2][2][2], Q[2][2][2], R[2][2][2];
+#pragma unroll
+        for (int gx = 0; gx < 2; gx++)
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const double x0 = c[2 * k], x1 = c[2 * k + 1];
+            A[gx][k & 1][k >> 1] = pair[0][gx] ? x0 + x1 : (gx ? x1 : x0);
+          }
+#pragma unroll
+        for (int gx = 0; gx < 2; gx++)
+#pragma unroll
+          for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+            for (int az = 0; az < 2; az++)
+              Q[gx][gy][az] = pair[1][gy] ? A[gx][0][az] + A[gx][1][az] : A[gx][gy][az];
+#pragma unroll
+        for (int gx = 0; gx < 2; gx++)
+#pragma unroll
+          for (int gy = 0; gy < 2; gy++)
+#pragma unroll
+            for (int gz = 0; gz < 2; gz++)
+              R[gx][gy][gz] = pair[2][gz] ? Q[gx][gy][0] + Q[gx][gy][1] : Q[gx][gy][gz];
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+          const bool ex = (!(m & 1) || (f & 1u)) && (!(m & 2) || (f & 2u)) && (!(m & 4) || (f & 4u));
+          if (!ex) continue;
+          // copy m's side on each split axis
+          const bool gx = (wrap & 1u) && (m & 1), gy = (wrap & 2u) && (m & 2), gz = (wrap & 4u) && (m & 4);
+          double rz[2][2];
+#pragma unroll
+          for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++) rz[x][y] = gz ? R[x][y][1] : R[x][y][0];
+          const double ry0 = gy ? rz[0][1] : rz[0][0], ry1 = gy ? rz[1][1] : rz[1][0];
+          lds[pp[m]] = gx ? ry1 : ry0;
+        }
       }
     }
   }

@@ -340,7 +340,7 @@ bool stream_ok(const faces_plan *p)
 template <int N, int S>
 int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t st)
 {
-  if (p->flow && mode <= 1 && !stream_big(p)) {
+  if (p->flow && !stream_big(p)) {
     // one item per block, items in memory order (faces_flow.hpp)
     FlowItems F;
     F.my = p->my;
@@ -349,6 +349,7 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t s
     F.nitems = p->my * p->mz * F.nseg;
     F.field_bytes = (unsigned)(8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps));
     F.xcd_remap = p->xcd_remap;
+    F.nbr_mask = p->mask;
     unsigned nb = (unsigned)F.nitems;
     if (F.xcd_remap == 2) {  // the XCD with the largest share of a plane sets the grid
       const int per = p->my * F.nseg;
@@ -362,12 +363,16 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t s
     // 16-byte OWN loads need a 16-byte aligned field; mode 1 skips the rank
     // surface (the staged step's interior sums)
     const bool w16 = p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0;
-    if (mode == 1) {
-      if (w16) k_flow<N, S, true, true><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
-      else k_flow<N, S, false, true><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
+    const dim3 grid(nb), blk(kStreamThreads);
+    if (mode == 2) {
+      if (w16) k_flow<N, S, true, 2><<<grid, blk, lds, st>>>(G, F);
+      else k_flow<N, S, false, 2><<<grid, blk, lds, st>>>(G, F);
+    } else if (mode == 1) {
+      if (w16) k_flow<N, S, true, 1><<<grid, blk, lds, st>>>(G, F);
+      else k_flow<N, S, false, 1><<<grid, blk, lds, st>>>(G, F);
     } else {
-      if (w16) k_flow<N, S, true><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
-      else k_flow<N, S, false><<<dim3(nb), dim3(kStreamThreads), lds, st>>>(G, F);
+      if (w16) k_flow<N, S, true, 0><<<grid, blk, lds, st>>>(G, F);
+      else k_flow<N, S, false, 0><<<grid, blk, lds, st>>>(G, F);
     }
     FACES_HIP(hipGetLastError());
     return FACES_OK;
@@ -475,7 +480,7 @@ int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, hipStre
 int launch_stream(const faces_plan *p, double *u, int mode, hipStream_t st)
 {
   const Geom G = make_geom(p, u);
-  const int S = (p->flow && mode <= 1 && !stream_big(p)) ? flow_seg(p) : stream_seg(p);
+  const int S = (p->flow && !stream_big(p)) ? flow_seg(p) : stream_seg(p);
   if (S == 0) return fail(FACES_ERR_ARG, "no streaming item length for n=%d mx=%d", p->n, p->mx);
   switch (p->n) {
     case 2: return launch_stream_n<2>(p, G, S, mode, st);

@@ -230,6 +230,7 @@ def test_open_rank_streaming_bitexact(torch_cuda, walk, rows, bpc, shape, seg, p
     d = to_dev(torch_cuda, u)
     f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.OPEN)
     f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+    f.set_option(fa.OPT_FLOW, 0)  # the streaming kernel's walks
     f.set_option(fa.OPT_SEG, seg)
     f.set_option(fa.OPT_STREAM_WALK, walk)
     f.set_option(fa.OPT_STREAM_ROWS, rows)
@@ -358,6 +359,7 @@ def test_rank_positions_open_streaming(torch_cuda, walk, rows, bpc, seg, planes)
         f = fa.Faces(rank, 3, 3, 3, mx, my, mz, n, boundary=fa.OPEN)
         f.set_path(fa.PATH_FUSED)
         f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+        f.set_option(fa.OPT_FLOW, 0)  # the streaming kernel's walks
         f.set_option(fa.OPT_SEG, seg)
         f.set_option(fa.OPT_STREAM_WALK, walk)
         f.set_option(fa.OPT_STREAM_ROWS, rows)
@@ -367,6 +369,58 @@ def test_rank_positions_open_streaming(torch_cuda, walk, rows, bpc, seg, planes)
         got = to_host(torch_cuda, d)
         f.close()
         assert_same(got, want, shape)
+
+
+@pytest.mark.parametrize("flow,xcd", [(2, 2), (1, 1), (2, 0)])
+@pytest.mark.parametrize("shape,seg", [((7, 12, 4, 3), 0), ((5, 10, 3, 2), 5), ((8, 4, 3, 5), 0),
+                                       ((4, 16, 2, 3), 0), ((6, 5, 2, 2), 1), ((2, 4, 1, 3), 0),
+                                       ((12, 2, 2, 2), 0)])
+def test_open_rank_flow_bitexact(torch_cuda, flow, xcd, shape, seg):
+    """The reference's own mode (open boundary, 1x1x1 rank grid) on the
+    one-item kernel: each rank-surface node sums only the copies on its side;
+    two shares equal the open sweep."""
+    n, mx, my, mz = shape
+    print("case open flow", flow, xcd, shape, seg, flush=True)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=818)
+    d = to_dev(torch_cuda, u)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.OPEN)
+    f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+    f.set_option(fa.OPT_FLOW, flow)
+    f.set_option(fa.OPT_XCD, xcd)
+    f.set_option(fa.OPT_SEG, seg)
+    for _ in range(2):
+        f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    assert_same(got, run_oracle(u, shape, periodic=False, nshare=2), shape)
+
+
+@pytest.mark.parametrize("flow,xcd,seg", [(2, 2, 0), (1, 1, 3), (2, 0, 1)])
+def test_rank_positions_open_flow(torch_cuda, flow, xcd, seg):
+    """Every rank position of a 3x3x3 open decomposition through the
+    one-item kernel (neighbours as self halos): bit-equal to the staged
+    26-message form; the staged path's interior sums (the same kernel with
+    the rank surface skipped) plus its surface sums give the same bits."""
+    shape = (7, 10, 4, 3)
+    n, mx, my, mz = shape
+    print("case rank positions flow", flow, xcd, seg, flush=True)
+    for rank in range(27):
+        mask = brute.mask_for_rank(rank, 3, 3, 3)
+        u = oracle.new_field(*shape, fa.FILL_RAND, seed=200 + rank)
+        want = u.copy()
+        oracle.share_staged(want, *shape, nbr_mask=mask)
+        for path in (fa.PATH_FUSED, fa.PATH_STAGED_DIRECT):
+            d = to_dev(torch_cuda, u)
+            f = fa.Faces(rank, 3, 3, 3, mx, my, mz, n, boundary=fa.OPEN)
+            f.set_path(path)
+            f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+            f.set_option(fa.OPT_FLOW, flow)
+            f.set_option(fa.OPT_XCD, xcd)
+            f.set_option(fa.OPT_SEG, seg)
+            f.share(d)
+            got = to_host(torch_cuda, d)
+            f.close()
+            assert_same(got, want, shape)
 
 
 def test_neighbor_table(torch_cuda):
