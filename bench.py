@@ -201,6 +201,17 @@ def served_leg(torch, fa, nreq, args_nstreams=2, bpc=0):
     # capacity: every request queued at once
     _, cap_st = srv.run_trace([0.0] * nreq, shapes, fields)
     srv.close()
+    # launches overlapping on the device: up to 400 small requests, one per
+    # launch, queued at once, through one launch slot and through two
+    small_idx = [i for i, sh in enumerate(shapes) if sh != LARGE_SHAPE][:400]
+    overlap = {}
+    for ns in (1, 2):
+        s1 = Server(footprint_cap=cap, max_batch=1, nstreams=ns)
+        s1.run_trace([0.0] * 4, [shapes[i] for i in small_idx[:4]], [fields[i] for i in small_idx[:4]])
+        _, st1 = s1.run_trace([0.0] * len(small_idx), [shapes[i] for i in small_idx],
+                              [fields[i] for i in small_idx])
+        overlap[ns] = st1["req_per_s"]
+        s1.close()
     # the same requests one plain launch each, in arrival order
     plans = {}
     stream = torch.cuda.current_stream()
@@ -230,7 +241,9 @@ def served_leg(torch, fa, nreq, args_nstreams=2, bpc=0):
             "GBs": round(nbytes / st["wall_s"] / 1e9, 2),
             "capacity_req_per_s": round(cap_st["req_per_s"], 1),
             "per_request_launch_req_per_s": round(nreq / per_req_s, 1),
-            "capacity_vs_per_request_launch": round(cap_st["req_per_s"] * per_req_s / nreq, 3)}
+            "capacity_vs_per_request_launch": round(cap_st["req_per_s"] * per_req_s / nreq, 3),
+            "one_request_per_launch_req_per_s": {"1_slot": round(overlap[1], 1), "2_slots": round(overlap[2], 1),
+                                                 "requests": len(small_idx)}}
 
 
 def copy_reference_gbs(torch, nbytes, reps=20):

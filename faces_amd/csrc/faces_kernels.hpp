@@ -600,72 +600,126 @@ __global__ __launch_bounds__(256) void k_surface(Geom G, Bound B, const double *
 // ---------------------------------------------------------------------------
 // pack: the 26 messages (gpu/v-26-messages/Faces.cc:243-293)
 // ---------------------------------------------------------------------------
-// Thread t of slot k copies message entry t.  dst[k] is the arena offset the
-// entry goes to: the send slot k, or -- fusing the self-halo exchange -- the
-// receive slot opposite(k); -1 skips the slot.
+// One launch, its blocks cut into ranges computed on the host: one range per
+// face message and one for the 20 edge and corner messages.  A block of a
+// y- or z-face range copies kPackFaceEntries consecutive entries (runs of n
+// or n^2 contiguous doubles in the field); a block of an x-face range stages
+// `xelem` whole boundary elements in LDS with contiguous loads and picks
+// their ix = 0 (or n-1) column, whose doubles lie n apart in the field (for
+// elements too large for LDS, xelem = 0: entries straight from the field);
+// edge and corner entries find their slot among the 20 small ones.
+// dst[k] is the arena offset entry 0 of slot k goes to: the send slot k, or
+// -- fusing the self-halo exchange -- the receive slot opposite(k); -1 skips
+// the slot.
+
+constexpr int kPackFaceEntries = 1024;
 
 struct PackDesc {
-  long src_off[27];  // prefix sums of slot sizes (the packing index space)
+  long src_off[27];  // prefix sums of slot sizes (entries)
   long dst[26];
+  int bstart[8];     // block prefix: ranges 0..5 the faces, 6 the edges and corners
+  int xelem;         // elements per block of an x-face range (0: no LDS staging)
 };
+
+// Field index of entry e of slot k (the reference's pack maps, Faces.cc:243-293).
+template <int N>
+__device__ __forceinline__ long pack_index(const Geom &G, int k, long e)
+{
+  const int n = nval<N>(G.n), nm1 = n - 1;
+  int ix, iy, iz, jx, jy, jz;
+  if (k < 6) {
+    const int s = k & 1;
+    const int a = (int)(e % n);
+    const int b = (int)((e / n) % n);
+    const long rest = e / ((long)n * n);
+    if (k < 2) {  // x-face (iy,iz,jy,jz)
+      iy = a; iz = b; jy = (int)(rest % G.my); jz = (int)(rest / G.my);
+      ix = s ? nm1 : 0; jx = s ? G.mx - 1 : 0;
+    } else if (k < 4) {  // y-face (ix,iz,jx,jz)
+      ix = a; iz = b; jx = (int)(rest % G.mx); jz = (int)(rest / G.mx);
+      iy = s ? nm1 : 0; jy = s ? G.my - 1 : 0;
+    } else {  // z-face (ix,iy,jx,jy)
+      ix = a; iy = b; jx = (int)(rest % G.mx); jy = (int)(rest / G.mx);
+      iz = s ? nm1 : 0; jz = s ? G.mz - 1 : 0;
+    }
+  } else if (k < 18) {
+    const int f = (k - 6) / 4, ee = (k - 6) % 4, a = ee & 1, b = ee >> 1;
+    const int i = (int)(e % n);
+    const int j = (int)(e / n);
+    if (f == 0) {  // x-edge: (sy, sz) = (a, b)
+      ix = i; jx = j;
+      iy = a ? nm1 : 0; jy = a ? G.my - 1 : 0;
+      iz = b ? nm1 : 0; jz = b ? G.mz - 1 : 0;
+    } else if (f == 1) {  // y-edge: (sx, sz) = (a, b)
+      iy = i; jy = j;
+      ix = a ? nm1 : 0; jx = a ? G.mx - 1 : 0;
+      iz = b ? nm1 : 0; jz = b ? G.mz - 1 : 0;
+    } else {  // z-edge: (sx, sy) = (a, b)
+      iz = i; jz = j;
+      ix = a ? nm1 : 0; jx = a ? G.mx - 1 : 0;
+      iy = b ? nm1 : 0; jy = b ? G.my - 1 : 0;
+    }
+  } else {
+    const int c = k - 18, sx = c & 1, sy = (c >> 1) & 1, sz = c >> 2;
+    ix = sx ? nm1 : 0; jx = sx ? G.mx - 1 : 0;
+    iy = sy ? nm1 : 0; jy = sy ? G.my - 1 : 0;
+    iz = sz ? nm1 : 0; jz = sz ? G.mz - 1 : 0;
+  }
+  const long n3 = (long)n * n * n;
+  return ix + (long)n * (iy + (long)n * iz) + jx * n3 + jy * G.ys + jz * G.ps;
+}
 
 template <int N>
 __global__ __launch_bounds__(256) void k_pack(Geom G, const PackDesc P, double *__restrict__ arena)
 {
-  const int n = nval<N>(G.n);
-  const int nm1 = n - 1;
-  const long total = P.src_off[26];
+  const int n = nval<N>(G.n), n2 = n * n;
+  const long n3 = (long)n * n2;
   const double *__restrict__ u = G.u;
-  const long n3 = (long)n * n * n;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
-       t += (long)gridDim.x * blockDim.x) {
-    int k = 0;
-#pragma unroll 1
-    while (k < 25 && t >= P.src_off[k + 1]) k++;
-    const long dk = P.dst[k];
-    if (dk < 0) continue;
-    const long e = t - P.src_off[k];
-    int ix, iy, iz, jx, jy, jz;
-    if (k < 6) {
-      const int s = k & 1;
-      const int a = (int)(e % n);
-      const int b = (int)((e / n) % n);
-      const long rest = e / ((long)n * n);
-      if (k < 2) {  // x-face (iy,iz,jy,jz)
-        iy = a; iz = b; jy = (int)(rest % G.my); jz = (int)(rest / G.my);
-        ix = s ? nm1 : 0; jx = s ? G.mx - 1 : 0;
-      } else if (k < 4) {  // y-face (ix,iz,jx,jz)
-        ix = a; iz = b; jx = (int)(rest % G.mx); jz = (int)(rest / G.mx);
-        iy = s ? nm1 : 0; jy = s ? G.my - 1 : 0;
-      } else {  // z-face (ix,iy,jx,jy)
-        ix = a; iy = b; jx = (int)(rest % G.mx); jy = (int)(rest / G.mx);
-        iz = s ? nm1 : 0; jz = s ? G.mz - 1 : 0;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  int r = 0;
+  while (r < 6 && b >= P.bstart[r + 1]) r++;  // block-uniform
+  const int lb = b - P.bstart[r];
+  if (r < 6) {
+    const long dk = P.dst[r];
+    if (dk < 0) return;
+    const long cnt = P.src_off[r + 1] - P.src_off[r];
+    if (r < 2 && P.xelem > 0) {
+      // x face: whole elements through LDS, contiguous loads
+      extern __shared__ double lds[];
+      const int s = r & 1;
+      const int jx = s ? G.mx - 1 : 0, ix = s ? n - 1 : 0;
+      const int nel = G.my * G.mz;
+      const int e0 = lb * P.xelem, ecnt = min(P.xelem, nel - e0);
+      const int tot = ecnt * (int)n3;
+      for (int t = tid; t < tot; t += 256) {
+        const int e = t / (int)n3, q = t - e * (int)n3;
+        const int el = e0 + e, jy = el % G.my, jz = el / G.my;
+        lds[t] = u[jx * n3 + jy * G.ys + jz * G.ps + q];
       }
-    } else if (k < 18) {
-      const int f = (k - 6) / 4, ee = (k - 6) % 4, a = ee & 1, b = ee >> 1;
-      const int i = (int)(e % n);
-      const int j = (int)(e / n);
-      if (f == 0) {  // x-edge: (sy, sz) = (a, b)
-        ix = i; jx = j;
-        iy = a ? nm1 : 0; jy = a ? G.my - 1 : 0;
-        iz = b ? nm1 : 0; jz = b ? G.mz - 1 : 0;
-      } else if (f == 1) {  // y-edge: (sx, sz) = (a, b)
-        iy = i; jy = j;
-        ix = a ? nm1 : 0; jx = a ? G.mx - 1 : 0;
-        iz = b ? nm1 : 0; jz = b ? G.mz - 1 : 0;
-      } else {  // z-edge: (sx, sy) = (a, b)
-        iz = i; jz = j;
-        ix = a ? nm1 : 0; jx = a ? G.mx - 1 : 0;
-        iy = b ? nm1 : 0; jy = b ? G.my - 1 : 0;
+      __syncthreads();
+      for (int t = tid; t < ecnt * n2; t += 256) {
+        const int e = t / n2, q = t - e * n2;  // q = iy + n*iz
+        arena[dk + (long)(e0 + e) * n2 + q] = lds[e * (int)n3 + q * n + ix];
       }
-    } else {
-      const int c = k - 18, sx = c & 1, sy = (c >> 1) & 1, sz = c >> 2;
-      ix = sx ? nm1 : 0; jx = sx ? G.mx - 1 : 0;
-      iy = sy ? nm1 : 0; jy = sy ? G.my - 1 : 0;
-      iz = sz ? nm1 : 0; jz = sz ? G.mz - 1 : 0;
+      return;
     }
-    arena[dk + e] = u[ix + (long)n * (iy + (long)n * iz) + jx * n3 + jy * G.ys + jz * G.ps];
+#pragma unroll
+    for (int it = 0; it < kPackFaceEntries / 256; it++) {
+      const long e = (long)lb * kPackFaceEntries + it * 256 + tid;
+      if (e < cnt) arena[dk + e] = u[pack_index<N>(G, r, e)];
+    }
+    return;
   }
+  // edges and corners
+  const long t = P.src_off[6] + (long)lb * 256 + tid;
+  if (t >= P.src_off[26]) return;
+  int k = 6;
+#pragma unroll 1
+  while (k < 25 && t >= P.src_off[k + 1]) k++;
+  const long dk = P.dst[k];
+  if (dk < 0) return;
+  const long e = t - P.src_off[k];
+  arena[dk + e] = u[pack_index<N>(G, k, e)];
 }
 
 // Self-halo exchange: recv[k] <- send[opposite(k)] for every existing
