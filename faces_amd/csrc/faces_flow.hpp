@@ -44,13 +44,25 @@ struct FlowItems {
 // rank: each side of the rank surface is a node of its own, which sums its
 // copies on that side, and those across the surface where the rank has a
 // neighbour there (stood in for by the self halo), x pairs, then y, then z.
-// W16: the OWN runs are loaded 16 bytes per lane (the field must be 16-byte
-// aligned).  An element's run [b, b + (n-1)n^2) has even length; it takes
-// (n-1)n^2/2 + 1 pair slots, of which an even b uses the first (n-1)n^2/2 and
-// an odd b all, starting one double early and ending one late (the halves
-// outside the run are dropped on the way to LDS).  Stores stay 8 bytes per
-// lane: a pair may hold one location this item owns and one it must not write.
-template <int N, int S, bool W16, int MODE = 0>
+// LW (load and store widths of the OWN region; the partner regions are moved
+// 8 bytes per lane):
+//   0  8 bytes per lane, loads and stores.
+//   1  16-byte loads on 16-byte boundaries (the field must be 16-byte
+//      aligned): an element's run [b, b + (n-1)n^2) has even length; it takes
+//      (n-1)n^2/2 + 1 pair slots, of which an even b uses the first
+//      (n-1)n^2/2 and an odd b all, starting one double early and ending one
+//      late (the halves outside the run are dropped on the way to LDS);
+//      stores 8 bytes per lane, since a pair may hold one location the item
+//      owns and one it must not write.
+//   2  16-byte loads and stores of exactly the locations the item owns,
+//      which form runs -- per element but the last, one run of (n-1)n
+//      doubles per plane iz < n-1 (its x-rows iy < n-1); per last element,
+//      one run of n-1 per x-row (ix < n-1) -- split into pairs from their
+//      starts, so a pair may start on any double (8-byte aligned); a run of
+//      odd length ends in a single moved 8 bytes per lane.  The rows iy = n-1
+//      and the last element's column ix = n-1, which the item neither sums
+//      nor stores, are not loaded.
+template <int N, int S, int LW, int MODE = 0>
 __global__ __launch_bounds__(kStreamThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_flow(Geom G, FlowItems I)
 {
@@ -143,11 +155,60 @@ void k_flow(Geom G, FlowItems I)
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   constexpr int L = K * n2;               // OWN run of one element (even)
   constexpr int PE = L / 2 + 1;           // pair slots per element (W16)
-  constexpr int JW = W16 ? (S * PE + kStreamThreads - 1) / kStreamThreads : 1;
+  constexpr int JW = LW == 1 ? (S * PE + kStreamThreads - 1) / kStreamThreads : 1;
   double v[J];
   u32x4 w[JW];
-  unsigned wq[JW];  // W16: LDS position of the pair's high half | bit 16: low valid | bit 17: high valid
-  if constexpr (W16) {
+  unsigned wq[JW];  // LW 1: LDS position of the pair's high half | bit 16: low valid | bit 17: high valid
+  // LW 2: pair slots over the owned runs, and singles ending odd runs
+  constexpr int RP = (K * n) / 2;                          // pairs per plane run
+  constexpr int PNL = K * RP;                              // pairs of an element but the last
+  constexpr int PRW = K / 2, PRW1 = PRW > 0 ? PRW : 1;    // pairs per row run of the last element
+  constexpr int NPR = (S - 1) * PNL + K * K * PRW;
+  constexpr int NSG = (K & 1) ? K * K : 0;                 // singles of the last element (n even)
+  constexpr int JR = (LW == 2 && NPR > 0) ? (NPR + kStreamThreads - 1) / kStreamThreads : 1;
+  constexpr int JG = (LW == 2 && NSG > 0) ? (NSG + kStreamThreads - 1) / kStreamThreads : 0;
+  unsigned roff[JR], rq[JR];             // byte offset (kOOB: none), LDS position of the low half
+  unsigned goff[JG > 0 ? JG : 1], gq[JG > 0 ? JG : 1];
+  u32x4 wr[JR];
+  double wg[JG > 0 ? JG : 1];
+  if constexpr (LW == 2) {
+#pragma unroll
+    for (int jr = 0; jr < JR; jr++) {
+      const int t = jr * kStreamThreads + tid;
+      roff[jr] = kOOB;
+      rq[jr] = 0;
+      if (t < NPR) {
+        int e, r0;
+        if (t < (S - 1) * PNL) {
+          e = t / PNL;
+          const int rem = t - e * PNL, iz = rem / RP;
+          r0 = iz * n2 + 2 * (rem - iz * RP);
+        } else {
+          const int t2 = t - (S - 1) * PNL, row = t2 / PRW1, iz = row / K, iy = row - iz * K;
+          e = S - 1;
+          r0 = iz * n2 + iy * n + 2 * (t2 - row * PRW1);
+        }
+        roff[jr] = rb[0] + ((unsigned)(jx0 + e) * n3 + (unsigned)r0) * 8;
+        rq[jr] = (unsigned)(e * (K * n2) + r0);
+      }
+      wr[jr] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, roff[jr], 0, 0);
+    }
+#pragma unroll
+    for (int jg = 0; jg < JG; jg++) {
+      const int t = jg * kStreamThreads + tid;
+      goff[jg] = kOOB;
+      gq[jg] = 0;
+      if (t < NSG) {
+        const int iz = t / K, iy = t - (t / K) * K, r = iz * n2 + iy * n + (K - 1);
+        goff[jg] = rb[0] + ((unsigned)(jx0 + S - 1) * n3 + (unsigned)r) * 8;
+        gq[jg] = (unsigned)((S - 1) * (K * n2) + r);
+      }
+      wg[jg] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, goff[jg], 0, 0));
+    }
+#pragma unroll
+    for (int j = JO; j < J; j++)
+      v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off[j], 0, 0));
+  } else if constexpr (LW == 1) {
     const unsigned row_d = rb[0] / 8;  // row base in doubles
 #pragma unroll
     for (int jw = 0; jw < JW; jw++) {
@@ -223,8 +284,21 @@ void k_flow(Geom G, FlowItems I)
   // across it are the wrap partners: segment 0, row jy = 0, plane jz = 0)
   const unsigned surf = MODE ? ((sg == 0 ? 1u : 0u) | (jy == 0 ? 2u : 0u) | (jz == 0 ? 4u : 0u)) : 0u;
 
-  if constexpr (W16) {
-    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  if constexpr (LW == 2) {
+#pragma unroll
+    for (int jr = 0; jr < JR; jr++) {
+      if (roff[jr] == kOOB) continue;
+      const u32x2 lo = {wr[jr][0], wr[jr][1]}, hi = {wr[jr][2], wr[jr][3]};
+      lds[rq[jr]] = __builtin_bit_cast(double, lo);
+      lds[rq[jr] + 1] = __builtin_bit_cast(double, hi);
+    }
+#pragma unroll
+    for (int jg = 0; jg < JG; jg++)
+      if (goff[jg] != kOOB) lds[gq[jg]] = wg[jg];
+#pragma unroll
+    for (int j = JO; j < J; j++) lds[j * kStreamThreads + tid] = v[j];
+  } else if constexpr (LW == 1) {
 #pragma unroll
     for (int jw = 0; jw < JW; jw++) {
       const unsigned q = wq[jw] & 0xffffu;
@@ -346,9 +420,19 @@ void k_flow(Geom G, FlowItems I)
   }
   __syncthreads();
   // ---- stores of the locations this item owns; the others go out of range
-  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  if constexpr (LW == 2) {
 #pragma unroll
-  for (int j = 0; j < J; j++) {
+    for (int jr = 0; jr < JR; jr++) {
+      const u32x2 lo = __builtin_bit_cast(u32x2, lds[rq[jr]]), hi = __builtin_bit_cast(u32x2, lds[rq[jr] + 1]);
+      const u32x4 x = {lo[0], lo[1], hi[0], hi[1]};
+      __builtin_amdgcn_raw_buffer_store_b128(x, rsrc, roff[jr], 0, 0);
+    }
+#pragma unroll
+    for (int jg = 0; jg < JG; jg++)
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, lds[gq[jg]]), rsrc, goff[jg], 0, 0);
+  }
+#pragma unroll
+  for (int j = (LW == 2 ? JO : 0); j < J; j++) {
     const u32x2 x = __builtin_bit_cast(u32x2, lds[j * kStreamThreads + tid]);
     __builtin_amdgcn_raw_buffer_store_b64(x, rsrc, (keep & (1u << j)) ? off[j] : kOOB, 0, 0);
   }

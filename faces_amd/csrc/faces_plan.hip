@@ -362,18 +362,26 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t s
     const size_t lds = (size_t)StreamShape<N, S>::P * sizeof(double);
     // 16-byte OWN loads need a 16-byte aligned field; mode 1 skips the rank
     // surface (the staged step's interior sums)
-    const bool w16 = p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0;
+    // load and store widths: flow 3 the owned runs in 16-byte pairs, flow 2
+    // 16-byte loads on 16-byte boundaries (a 16-byte aligned field; else 8
+    // bytes), flow 1 8 bytes per lane
+    const int lw = p->flow == 3 ? 2 : (p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0) ? 1 : 0;
     const dim3 grid(nb), blk(kStreamThreads);
+#define FACES_LAUNCH_FLOW(LW, MODE) k_flow<N, S, LW, MODE><<<grid, blk, lds, st>>>(G, F)
     if (mode == 2) {
-      if (w16) k_flow<N, S, true, 2><<<grid, blk, lds, st>>>(G, F);
-      else k_flow<N, S, false, 2><<<grid, blk, lds, st>>>(G, F);
+      if (lw == 2) FACES_LAUNCH_FLOW(2, 2);
+      else if (lw == 1) FACES_LAUNCH_FLOW(1, 2);
+      else FACES_LAUNCH_FLOW(0, 2);
     } else if (mode == 1) {
-      if (w16) k_flow<N, S, true, 1><<<grid, blk, lds, st>>>(G, F);
-      else k_flow<N, S, false, 1><<<grid, blk, lds, st>>>(G, F);
+      if (lw == 2) FACES_LAUNCH_FLOW(2, 1);
+      else if (lw == 1) FACES_LAUNCH_FLOW(1, 1);
+      else FACES_LAUNCH_FLOW(0, 1);
     } else {
-      if (w16) k_flow<N, S, true, 0><<<grid, blk, lds, st>>>(G, F);
-      else k_flow<N, S, false, 0><<<grid, blk, lds, st>>>(G, F);
+      if (lw == 2) FACES_LAUNCH_FLOW(2, 0);
+      else if (lw == 1) FACES_LAUNCH_FLOW(1, 0);
+      else FACES_LAUNCH_FLOW(0, 0);
     }
+#undef FACES_LAUNCH_FLOW
     FACES_HIP(hipGetLastError());
     return FACES_OK;
   }
@@ -752,10 +760,14 @@ int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, i
   // for n <= 6 (79x80x80 n=6 475 against 565)
   p->items_bpc = (n >= 7) ? 16 : 32;
   p->stream_planes = 0;
-  // The one-item kernel with 16-byte OWN loads for periodic steps: 60^3 n=7
-  // 249-253 us against 274 us for the streaming kernel's column walk, 64^3
-  // n=8 410 against 449, 128^3 n=4 413 against 477.
-  p->flow = 2;
+  // The one-item kernel for periodic steps and open ranks: 60^3 n=7 230-236
+  // us against 274 us for the streaming kernel's column walk, 64^3 n=8 410
+  // against 449, 128^3 n=4 413-426 against 477.  For odd n it moves exactly
+  // the owned locations in 16-byte pairs (flow 3; runs of even length: 60^3
+  // n=7 229 us against 230-236, 48^3 n=7 116 against 123, 100^3 n=5 412
+  // against 422); for even n, whose last-element rows end in singles, it
+  // loads on 16-byte boundaries (flow 2; 40^3 n=12 363 us against 416).
+  p->flow = (n % 2) ? 3 : 2;
 
   // Message sizes, gpu/v-26-messages/Faces.cc:44-47, 64-66.
   const long nn = (long)n * n;
@@ -921,7 +933,7 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       p->stream_planes = value ? 1 : 0;
       return FACES_OK;
     case FACES_OPT_FLOW:
-      if (value < 0 || value > 2) return fail(FACES_ERR_ARG, "one-item kernel mode %d outside [0,2]", value);
+      if (value < 0 || value > 3) return fail(FACES_ERR_ARG, "one-item kernel mode %d outside [0,3]", value);
       p->flow = value;
       return FACES_OK;
     default: return fail(FACES_ERR_ARG, "unknown option %d", option);
@@ -966,7 +978,7 @@ int faces_plan_tune(faces_plan *p, double *u, int reps, void *stream, double *ou
       p->seg_len = sl;
       if (flow_seg(p) == sl) segs.push_back(sl);
     }
-    for (int fl : {2, 1})
+    for (int fl : {3, 2, 1})
       for (int sl : segs)
         for (int x : {2, 1}) cands.push_back({FACES_VARIANT_STREAM, p->stream_walk, p->stream_rows, sl, p->stream_bpc, fl, x});
   }

@@ -170,13 +170,17 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *                            descriptor per jz plane (what fields of 4 GB and
  *                            more use; set, it applies to any field); get:
  *                            whether the plan addresses per plane
- *   FACES_OPT_FLOW           periodic steps with compute = 1 and the staged
- *                            step's interior sums (fields below 4 GB):
- *                            2 (default) = the one-item kernel, one
+ *   FACES_OPT_FLOW           periodic steps with compute = 1, open ranks and
+ *                            the staged step's interior sums (fields below
+ *                            4 GB): 2 (default for even n) = the one-item
+ *                            kernel, one
  *                            work item per block in memory order, OWN runs
  *                            loaded 16 bytes per lane (8 when the field is not
- *                            16-byte aligned); 1 = the same with 8-byte lanes;
- *                            0 = the STREAM walks above */
+ *                            16-byte aligned); 3 (default for odd n) = the
+ *                            same kernel loading and
+ *                            storing exactly the locations each item owns, in
+ *                            16-byte pairs that may start on any double; 1 =
+ *                            8-byte lanes; 0 = the STREAM walks above */
 #define FACES_OPT_VARIANT 0
 #define FACES_OPT_DIRECT_BLOCK 1
 #define FACES_OPT_SEG 2
