@@ -22,7 +22,8 @@ INCLUDE = os.path.join(ROOT, "include")
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["faces_plan.hip", "faces_batch.hip", "faces_serve.cpp", "faces_noalloc.hip"]
+HIP_SOURCES = ["faces_plan.hip", "faces_flow.hip", "faces_batch.hip", "faces_serve.cpp", "faces_noalloc.hip"]
+OBJDIR = os.path.join(PKG, "_build")
 
 COMMON_FLAGS = [
     "-std=c++17",
@@ -46,20 +47,27 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_library(verbose: bool = True) -> str:
+    """Compiles every translation unit to an object (in parallel: the kernel
+    instantiations of the plan and of the one-item kernel take minutes each),
+    then links them into one shared library."""
     os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(OBJDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libfaces_amd.so")
     tmp = out + ".tmp"
-    srcs = []
+    jobs = []
     for s in HIP_SOURCES:
         p = os.path.join(CSRC, s)
         if not os.path.exists(p):
             continue
-        if s.endswith(".cpp"):
-            srcs += ["-x", "hip", p]
-        else:
-            srcs.append(p)
-    cmd = [HIPCC, *DEVICE_FLAGS, *COMMON_FLAGS, "-shared", "-o", tmp, *srcs, "-lpthread"]
-    _run(cmd)
+        obj = os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o")
+        src = ["-x", "hip", p] if s.endswith(".cpp") else [p]
+        cmd = [HIPCC, *DEVICE_FLAGS, *COMMON_FLAGS, "-c", *src, "-o", obj]
+        print("+", " ".join(shlex.quote(c) for c in cmd), flush=True)
+        jobs.append((s, obj, subprocess.Popen(cmd)))
+    failed = [s for s, _, proc in jobs if proc.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, f"hipcc -c {' '.join(failed)}")
+    _run([HIPCC, *DEVICE_FLAGS, "-fPIC", "-shared", "-o", tmp, *[obj for _, obj, _ in jobs], "-lpthread"])
     os.replace(tmp, out)
     return out
 

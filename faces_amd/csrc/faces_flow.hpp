@@ -62,7 +62,13 @@ struct FlowItems {
 //      odd length ends in a single moved 8 bytes per lane.  The rows iy = n-1
 //      and the last element's column ix = n-1, which the item neither sums
 //      nor stores, are not loaded.
-template <int N, int S, int LW, int MODE = 0>
+// TREE (the order of a node's sum tree): 0 x pairs, then y, then z (the
+// 26-message step); 1 z pairs, then y, then x (the 6-message schedule's three
+// phases, gpu/v-6-messages-basic/Faces.cc:65-172, whose z phase sums the z
+// pairs, its y phase the pairs of those sums, its x phase the rest).  Tree 1
+// is tree 0 with the x and z axes exchanged: the copies and flags of a node
+// are relabelled (copy bits sx <-> sz) before the same code sums them.
+template <int N, int S, int LW, int MODE = 0, int TREE = 0>
 __global__ __launch_bounds__(kStreamThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_flow(Geom G, FlowItems I)
 {
@@ -313,7 +319,8 @@ void k_flow(Geom G, FlowItems I)
     for (int j = 0; j < J; j++) lds[j * kStreamThreads + tid] = v[j];
   }
   __syncthreads();
-  // ---- shared nodes: x pairs, then y, then z; the sum goes to every copy
+  // ---- shared nodes: x pairs, then y, then z (TREE 1: z, y, x); the sum
+  // goes to every copy
 #pragma unroll
   for (int i = 0; i < NN; i++) {
     const unsigned f = nflag[i];
@@ -321,11 +328,12 @@ void k_flow(Geom G, FlowItems I)
     // a node whose x pair (element 0 of the item only), y or z pair crosses
     // the rank surface is the surface sums'
     if (MODE && (surf & ((f & 6u) | ((f & 17u) == 17u ? 1u : 0u)))) continue;
-    const unsigned p0 = np[i][0] & 0xffff, p1 = np[i][0] >> 16;
-    const unsigned p2 = np[i][1] & 0xffff, p3 = np[i][1] >> 16;
-    const unsigned p4 = np[i][2] & 0xffff, p5 = np[i][2] >> 16;
-    const unsigned p6 = np[i][3] & 0xffff, p7 = np[i][3] >> 16;
-    const bool cx = f & 1u, cy = f & 2u, cz = f & 4u;
+    // copy c = sx | sy << 1 | sz << 2 at position p<c>; tree 1 exchanges sx and sz
+    const unsigned p0 = np[i][0] & 0xffff, p1 = TREE ? np[i][2] & 0xffff : np[i][0] >> 16;
+    const unsigned p2 = np[i][1] & 0xffff, p3 = TREE ? np[i][3] & 0xffff : np[i][1] >> 16;
+    const unsigned p4 = TREE ? np[i][0] >> 16 : np[i][2] & 0xffff, p5 = np[i][2] >> 16;
+    const unsigned p6 = TREE ? np[i][1] >> 16 : np[i][3] & 0xffff, p7 = np[i][3] >> 16;
+    const bool cx = f & (TREE ? 4u : 1u), cy = f & 2u, cz = f & (TREE ? 1u : 4u);
     const double a0 = cx ? lds[p0] + lds[p1] : lds[p0];
     double q0 = a0;
     if (cy) q0 = a0 + (cx ? lds[p2] + lds[p3] : lds[p2]);
@@ -364,9 +372,17 @@ void k_flow(Geom G, FlowItems I)
         if (!wrap) continue;
         // copy (sx | sy << 1 | sz << 2), side 1 = the partner across that axis;
         // all eight are read (positions of copies the node does not have lie
-        // in the footprint too, and go unused)
-        const unsigned pp[8] = {np[i][0] & 0xffff, np[i][0] >> 16, np[i][1] & 0xffff, np[i][1] >> 16,
-                                np[i][2] & 0xffff, np[i][2] >> 16, np[i][3] & 0xffff, np[i][3] >> 16};
+        // in the footprint too, and go unused).  Tree 1: the axes x and z
+        // exchanged in the copy numbering, the flags, the split axes and the
+        // neighbour mask, so that the levels below run z, y, x.
+        const unsigned pp[8] = {
+            np[i][0] & 0xffff, TREE ? np[i][2] & 0xffff : np[i][0] >> 16,
+            np[i][1] & 0xffff, TREE ? np[i][3] & 0xffff : np[i][1] >> 16,
+            TREE ? np[i][0] >> 16 : np[i][2] & 0xffff, np[i][2] >> 16,
+            TREE ? np[i][1] >> 16 : np[i][3] & 0xffff, np[i][3] >> 16};
+        const unsigned ft = TREE ? (f & 2u) | ((f & 1u) << 2) | ((f >> 2) & 1u) : f;
+        const unsigned wt = TREE ? (wrap & 2u) | ((wrap & 1u) << 2) | ((wrap >> 2) & 1u) : wrap;
+        const int nbr = TREE ? (I.nbr_mask & 12) | ((I.nbr_mask & 3) << 4) | ((I.nbr_mask >> 4) & 3) : I.nbr_mask;
         double c[8];
 #pragma unroll
         for (int m = 0; m < 8; m++) c[m] = lds[pp[m]];
@@ -377,7 +393,7 @@ void k_flow(Geom G, FlowItems I)
         for (int a = 0; a < 3; a++)
 #pragma unroll
           for (int sd = 0; sd < 2; sd++)
-            pair[a][sd] = (f & (1u << a)) && (!(wrap & (1u << a)) || (I.nbr_mask & (1 << (2 * a + sd))));
+            pair[a][sd] = (ft & (1u << a)) && (!(wt & (1u << a)) || (nbr & (1 << (2 * a + sd))));
         // the tree level by level for both sides of every axis
         double A[2][2][2], Q[2][2][2], R[2][2][2];
 #pragma unroll
@@ -403,10 +419,10 @@ void k_flow(Geom G, FlowItems I)
               R[gx][gy][gz] = pair[2][gz] ? Q[gx][gy][0] + Q[gx][gy][1] : Q[gx][gy][gz];
 #pragma unroll
         for (int m = 0; m < 8; m++) {
-          const bool ex = (!(m & 1) || (f & 1u)) && (!(m & 2) || (f & 2u)) && (!(m & 4) || (f & 4u));
+          const bool ex = (!(m & 1) || (ft & 1u)) && (!(m & 2) || (ft & 2u)) && (!(m & 4) || (ft & 4u));
           if (!ex) continue;
           // copy m's side on each split axis
-          const bool gx = (wrap & 1u) && (m & 1), gy = (wrap & 2u) && (m & 2), gz = (wrap & 4u) && (m & 4);
+          const bool gx = (wt & 1u) && (m & 1), gy = (wt & 2u) && (m & 2), gz = (wt & 4u) && (m & 4);
           double rz[2][2];
 #pragma unroll
           for (int x = 0; x < 2; x++)
@@ -437,5 +453,10 @@ void k_flow(Geom G, FlowItems I)
     __builtin_amdgcn_raw_buffer_store_b64(x, rsrc, (keep & (1u << j)) ? off[j] : kOOB, 0, 0);
   }
 }
+
+// Launches k_flow<n, S, lw, mode, tree> on nb blocks of kStreamThreads
+// (faces_flow.hip).  Tree 1 is compiled with load width 2 and modes 0 and 2.
+int launch_flow(int n, int S, int lw, int mode, int tree, const Geom &G, const FlowItems &F, unsigned nb,
+                hipStream_t st);
 
 }  // namespace faces

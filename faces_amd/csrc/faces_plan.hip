@@ -338,7 +338,7 @@ bool stream_ok(const faces_plan *p)
 }
 
 template <int N, int S>
-int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t st)
+int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, hipStream_t st)
 {
   if (p->flow && !stream_big(p)) {
     // one item per block, items in memory order (faces_flow.hpp)
@@ -359,32 +359,16 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t s
     } else if (F.xcd_remap == 1) {
       nb = 8u * (unsigned)((F.nitems + 7) / 8);
     }
-    const size_t lds = (size_t)StreamShape<N, S>::P * sizeof(double);
-    // 16-byte OWN loads need a 16-byte aligned field; mode 1 skips the rank
-    // surface (the staged step's interior sums)
     // load and store widths: flow 3 the owned runs in 16-byte pairs, flow 2
     // 16-byte loads on 16-byte boundaries (a 16-byte aligned field; else 8
-    // bytes), flow 1 8 bytes per lane
-    const int lw = p->flow == 3 ? 2 : (p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0) ? 1 : 0;
-    const dim3 grid(nb), blk(kStreamThreads);
-#define FACES_LAUNCH_FLOW(LW, MODE) k_flow<N, S, LW, MODE><<<grid, blk, lds, st>>>(G, F)
-    if (mode == 2) {
-      if (lw == 2) FACES_LAUNCH_FLOW(2, 2);
-      else if (lw == 1) FACES_LAUNCH_FLOW(1, 2);
-      else FACES_LAUNCH_FLOW(0, 2);
-    } else if (mode == 1) {
-      if (lw == 2) FACES_LAUNCH_FLOW(2, 1);
-      else if (lw == 1) FACES_LAUNCH_FLOW(1, 1);
-      else FACES_LAUNCH_FLOW(0, 1);
-    } else {
-      if (lw == 2) FACES_LAUNCH_FLOW(2, 0);
-      else if (lw == 1) FACES_LAUNCH_FLOW(1, 0);
-      else FACES_LAUNCH_FLOW(0, 0);
-    }
-#undef FACES_LAUNCH_FLOW
-    FACES_HIP(hipGetLastError());
-    return FACES_OK;
+    // bytes), flow 1 8 bytes per lane; the tree in z, y, x order (the
+    // 6-message schedule's) is compiled with the 16-byte pairs only
+    const int lw = (p->flow == 3 || tree)                                                  ? 2
+                   : (p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0) ? 1
+                                                                                   : 0;
+    return launch_flow(N, S, lw, mode, tree, G, F, nb, st);
   }
+  if (tree) return fail(FACES_ERR_ARG, "the z, y, x tree is the one-item kernel's only");
   StreamItems I = {};
   I.my = p->my;
   I.mz = p->mz;
@@ -455,53 +439,53 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, hipStream_t s
 }
 
 template <int N>
-int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, hipStream_t st)
+int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, int tree, hipStream_t st)
 {
   switch (S) {
-    case 1: return launch_stream_ns<N, 1>(p, G, mode, st);
-    case 2: return launch_stream_ns<N, 2>(p, G, mode, st);
-    case 3: return launch_stream_ns<N, 3>(p, G, mode, st);
-    case 4: return launch_stream_ns<N, 4>(p, G, mode, st);
+    case 1: return launch_stream_ns<N, 1>(p, G, mode, tree, st);
+    case 2: return launch_stream_ns<N, 2>(p, G, mode, tree, st);
+    case 3: return launch_stream_ns<N, 3>(p, G, mode, tree, st);
+    case 4: return launch_stream_ns<N, 4>(p, G, mode, tree, st);
     default: break;
   }
   if constexpr (N >= 5 && N <= 8) {
-    if (S == 5) return launch_stream_ns<N, 5>(p, G, mode, st);
+    if (S == 5) return launch_stream_ns<N, 5>(p, G, mode, tree, st);
   }
   if constexpr (N == 5) {
-    if (S == 10) return launch_stream_ns<N, 10>(p, G, mode, st);
+    if (S == 10) return launch_stream_ns<N, 10>(p, G, mode, tree, st);
   }
   if constexpr (N <= 7) {
-    if (S == 6) return launch_stream_ns<N, 6>(p, G, mode, st);
+    if (S == 6) return launch_stream_ns<N, 6>(p, G, mode, tree, st);
   }
   if constexpr (N <= 6) {
-    if (S == 8) return launch_stream_ns<N, 8>(p, G, mode, st);
+    if (S == 8) return launch_stream_ns<N, 8>(p, G, mode, tree, st);
   }
   if constexpr (N <= 4) {
-    if (S == 16) return launch_stream_ns<N, 16>(p, G, mode, st);
+    if (S == 16) return launch_stream_ns<N, 16>(p, G, mode, tree, st);
   }
   if constexpr (N == 2) {
-    if (S == 32) return launch_stream_ns<N, 32>(p, G, mode, st);
+    if (S == 32) return launch_stream_ns<N, 32>(p, G, mode, tree, st);
   }
   return fail(FACES_ERR_ARG, "streaming kernel has no n=%d S=%d specialisation", N, S);
 }
 
-int launch_stream(const faces_plan *p, double *u, int mode, hipStream_t st)
+int launch_stream(const faces_plan *p, double *u, int mode, hipStream_t st, int tree = 0)
 {
   const Geom G = make_geom(p, u);
   const int S = (p->flow && !stream_big(p)) ? flow_seg(p) : stream_seg(p);
   if (S == 0) return fail(FACES_ERR_ARG, "no streaming item length for n=%d mx=%d", p->n, p->mx);
   switch (p->n) {
-    case 2: return launch_stream_n<2>(p, G, S, mode, st);
-    case 3: return launch_stream_n<3>(p, G, S, mode, st);
-    case 4: return launch_stream_n<4>(p, G, S, mode, st);
-    case 5: return launch_stream_n<5>(p, G, S, mode, st);
-    case 6: return launch_stream_n<6>(p, G, S, mode, st);
-    case 7: return launch_stream_n<7>(p, G, S, mode, st);
-    case 8: return launch_stream_n<8>(p, G, S, mode, st);
-    case 9: return launch_stream_n<9>(p, G, S, mode, st);
-    case 10: return launch_stream_n<10>(p, G, S, mode, st);
-    case 11: return launch_stream_n<11>(p, G, S, mode, st);
-    case 12: return launch_stream_n<12>(p, G, S, mode, st);
+    case 2: return launch_stream_n<2>(p, G, S, mode, tree, st);
+    case 3: return launch_stream_n<3>(p, G, S, mode, tree, st);
+    case 4: return launch_stream_n<4>(p, G, S, mode, tree, st);
+    case 5: return launch_stream_n<5>(p, G, S, mode, tree, st);
+    case 6: return launch_stream_n<6>(p, G, S, mode, tree, st);
+    case 7: return launch_stream_n<7>(p, G, S, mode, tree, st);
+    case 8: return launch_stream_n<8>(p, G, S, mode, tree, st);
+    case 9: return launch_stream_n<9>(p, G, S, mode, tree, st);
+    case 10: return launch_stream_n<10>(p, G, S, mode, tree, st);
+    case 11: return launch_stream_n<11>(p, G, S, mode, tree, st);
+    case 12: return launch_stream_n<12>(p, G, S, mode, tree, st);
     default: return fail(FACES_ERR_ARG, "streaming kernel has no n=%d specialisation", p->n);
   }
 }
@@ -1126,6 +1110,13 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
   if (path == FACES_PATH_SIX) {
     // z, y, x phases; self halos on the sides where a neighbour exists
     const int mask = (p->boundary == FACES_BOUNDARY_PERIODIC) ? 63 : p->mask;
+    if (compute && p->flow && !stream_big(p) && flow_seg(p) > 0) {
+      // One launch: a node's three phases are its sum tree with the levels
+      // in z, y, x order, which the one-item kernel evaluates on its copies
+      // (an open rank with all six face neighbours sums the periodic wrap
+      // partners, as in the fused 26-message step).
+      return launch_stream(p, u, mask == 63 ? 0 : 2, s, 1);
+    }
     Geom G = make_geom(p, u);
     const int m[3] = {p->mx, p->my, p->mz};
     for (int axis = 2; axis >= 0; axis--) {

@@ -57,7 +57,10 @@ extern "C" {
                                       (gpu/v-6-messages-basic/Faces.cc:65-172);
                                       its summation tree is z-first, so random
                                       fields differ from the other paths by
-                                      rounding */
+                                      rounding.  compute = 1 takes one launch
+                                      (the one-item kernel, tree z, y, x) where
+                                      FACES_OPT_FLOW is not 0 and the field is
+                                      below 4 GB; else three phase launches */
 
 /* Field generators (faces_fill). */
 #define FACES_FILL_INT 0  /* integer-valued, uniform in [0,1023] */

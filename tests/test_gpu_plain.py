@@ -657,6 +657,53 @@ def test_six_open_ranks_bitexact(torch_cuda, lgrid, rank, kind):
         assert_same(got, ref, shape)
 
 
+def _six_gpu(torch_cuda, u, shape, boundary, rank, lgrid, flow=None, nshare=1):
+    n, mx, my, mz = shape
+    d = to_dev(torch_cuda, u)
+    f = fa.Faces(rank, *lgrid, mx, my, mz, n, boundary=boundary)
+    f.set_path(fa.PATH_SIX)
+    if flow is not None:
+        f.set_option(fa.OPT_FLOW, flow)
+    for _ in range(nshare):
+        f.share(d)
+    out = to_host(torch_cuda, d)
+    f.close()
+    return out
+
+
+@pytest.mark.parametrize("shape", [(5, 5, 3, 4), (4, 6, 2, 3), (7, 10, 3, 2), (12, 2, 2, 1)])
+def test_six_one_launch_all_ranks_bitexact(torch_cuda, shape):
+    """With compute = 1 the 6-message schedule runs as one launch of the
+    one-item kernel with its tree in z, y, x order: every rank position of a
+    3x3x3 grid (each neighbour mask), random fields, bit for bit against the
+    oracle's three phases and against the three phase kernels."""
+    n, mx, my, mz = shape
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=503)
+    for rank in range(27):
+        print("case six one launch", shape, rank, flush=True)
+        got = _six_gpu(torch_cuda, u, shape, fa.OPEN, rank, (3, 3, 3))
+        want = u.copy()
+        oracle.share_six(want, *shape, nbr_mask=brute.mask_for_rank(rank, 3, 3, 3))
+        assert_same(got, want, shape)
+        if rank in (0, 13, 26):
+            phases = _six_gpu(torch_cuda, u, shape, fa.OPEN, rank, (3, 3, 3), flow=0)
+            assert np.array_equal(got.view(np.uint64), phases.view(np.uint64))
+
+
+@pytest.mark.parametrize("flow", [0, 1, 2, 3])
+def test_six_periodic_flow_options_bitexact(torch_cuda, flow):
+    """Periodic 6-message schedule at every FACES_OPT_FLOW setting (0: the
+    three phase kernels; else one launch, tree z, y, x): same bits."""
+    shape = (7, 10, 6, 4)
+    print("case six periodic flow", flow, flush=True)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=504)
+    got = _six_gpu(torch_cuda, u, shape, fa.PERIODIC, 0, (1, 1, 1), flow=flow, nshare=2)
+    want = u.copy()
+    for _ in range(2):
+        oracle.share_six(want, *shape, nbr_mask=63)
+    assert_same(got, want, shape)
+
+
 def test_six_c2_size_integer(torch_cuda):
     """60^3 n=7 integer field, 3 shares of PATH_SIX equal the oracle sweep."""
     shape = (7, 60, 60, 60)
