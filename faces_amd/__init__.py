@@ -34,7 +34,8 @@ OPEN, PERIODIC = 0, 1
 PATH_AUTO, PATH_FUSED, PATH_STAGED, PATH_STAGED_DIRECT, PATH_SIX = 0, 1, 2, 3, 4
 VARIANT_AUTO, VARIANT_DIRECT, VARIANT_LDS, VARIANT_STREAM, VARIANT_ITEMS = 0, 1, 2, 3, 4
 (OPT_VARIANT, OPT_DIRECT_BLOCK, OPT_SEG, OPT_XCD, OPT_LDS_BLOCK, OPT_STREAM_BPC,
- OPT_STREAM_WALK, OPT_ACTIVE_KERNEL, OPT_STREAM_ROWS, OPT_STREAM_PLANES, OPT_FLOW) = range(11)
+ OPT_STREAM_WALK, OPT_ACTIVE_KERNEL, OPT_STREAM_ROWS, OPT_STREAM_PLANES, OPT_FLOW,
+ OPT_SURFACE_BLOCKS) = range(12)
 FILL_INT, FILL_RAND, FILL_REF = 0, 1, 2
 
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_ADMISSION, ERR_NODEV, ERR_TIMEOUT = (

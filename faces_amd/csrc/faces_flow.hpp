@@ -68,9 +68,14 @@ struct FlowItems {
 // pairs, its y phase the pairs of those sums, its x phase the rest).  Tree 1
 // is tree 0 with the x and z axes exchanged: the copies and flags of a node
 // are relabelled (copy bits sx <-> sz) before the same code sums them.
+// LDS words of the MODE 1 marks (one bit per footprint position), placed
+// after the footprint's StreamShape<N, S>::P doubles.
+template <int N, int S>
+constexpr int kFlowMarkWords = (StreamShape<N, S>::P + 31) / 32;
+
 template <int N, int S, int LW, int MODE = 0, int TREE = 0>
 __global__ __launch_bounds__(kStreamThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
-void k_flow(Geom G, FlowItems I)
+void k_flow(Geom G, FlowItems I, const SurfWork W)
 {
   using SH = StreamShape<N, S>;
   constexpr int n = N, K = N - 1, n2 = N * N;
@@ -80,20 +85,33 @@ void k_flow(Geom G, FlowItems I)
   extern __shared__ double lds[];
   const int tid = threadIdx.x;
 
+  // MODE 1: the surface blocks of the launch sum the rank surface meanwhile
+  unsigned bid = blockIdx.x;
+  if constexpr (MODE == 1) {
+    if (bid - W.sfirst < W.nblk) {
+      const long total = W.S.cz + W.S.cy + W.S.cx;
+      const long stride = (long)W.nblk * kStreamThreads;
+      for (long t = (long)(bid - W.sfirst) * kStreamThreads + tid; t < total; t += stride)
+        surface_node<N>(G, W.B, W.recv, W.sl, W.S, t);
+      return;
+    }
+    bid -= W.item0;
+  }
+
   // ---- the item
-  int it = blockIdx.x;
+  int it = bid;
   if (I.xcd_remap == 2) {
     // The x-th eighth of the items of every plane, planes in order: an
     // item's z partner is one plane's share (an eighth of a plane) back in
     // this XCD's walk, its y partner nseg items back.
-    const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const int xcd = bid & 7, k = bid >> 3;
     const int per = G.my * I.nseg;  // items per plane
     const int i0 = xcd * per / 8, cnt = (xcd + 1) * per / 8 - i0;
     if (cnt == 0 || k >= cnt * G.mz) return;
     const int jzk = k / cnt;
     it = jzk * per + i0 + (k - jzk * cnt);
   } else if (I.xcd_remap == 1) {
-    const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const int xcd = bid & 7, k = bid >> 3;
     const int q8 = I.nitems >> 3, rem8 = I.nitems & 7;
     if (k >= q8 + (xcd < rem8 ? 1 : 0)) return;
     it = xcd * q8 + (xcd < rem8 ? xcd : rem8) + k;
@@ -318,6 +336,13 @@ void k_flow(Geom G, FlowItems I)
 #pragma unroll
     for (int j = 0; j < J; j++) lds[j * kStreamThreads + tid] = v[j];
   }
+  // MODE 1: one bit per LDS position, set for the copies of the rank-surface
+  // nodes, which the item leaves to the surface sums and does not store
+  // either: the surface sums run at the same time (the surface blocks)
+  unsigned *const smark = reinterpret_cast<unsigned *>(lds + SH::P);
+  if (MODE == 1 && surf) {
+    for (int w = tid; w < kFlowMarkWords<N, S>; w += kStreamThreads) smark[w] = 0u;
+  }
   __syncthreads();
   // ---- shared nodes: x pairs, then y, then z (TREE 1: z, y, x); the sum
   // goes to every copy
@@ -327,7 +352,18 @@ void k_flow(Geom G, FlowItems I)
     if (!(f & 8u)) continue;
     // a node whose x pair (element 0 of the item only), y or z pair crosses
     // the rank surface is the surface sums'
-    if (MODE && (surf & ((f & 6u) | ((f & 17u) == 17u ? 1u : 0u)))) continue;
+    if (MODE && (surf & ((f & 6u) | ((f & 17u) == 17u ? 1u : 0u)))) {
+      if constexpr (MODE == 1) {
+        const unsigned fc = f & 7u;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          if ((c & ~fc) != 0u) continue;  // copies the node has
+          const unsigned pc = (c & 1) ? np[i][c >> 1] >> 16 : np[i][c >> 1] & 0xffff;
+          atomicOr(&smark[pc >> 5], 1u << (pc & 31));
+        }
+      }
+      continue;
+    }
     // copy c = sx | sy << 1 | sz << 2 at position p<c>; tree 1 exchanges sx and sz
     const unsigned p0 = np[i][0] & 0xffff, p1 = TREE ? np[i][2] & 0xffff : np[i][0] >> 16;
     const unsigned p2 = np[i][1] & 0xffff, p3 = TREE ? np[i][3] & 0xffff : np[i][1] >> 16;
@@ -436,16 +472,46 @@ void k_flow(Geom G, FlowItems I)
   }
   __syncthreads();
   // ---- stores of the locations this item owns; the others go out of range
+  // (MODE 1: the copies of rank-surface nodes too)
+  const bool excl = MODE == 1 && surf;
+  auto marked = [&](unsigned pos) { return ((smark[pos >> 5] >> (pos & 31)) & 1u) != 0u; };
   if constexpr (LW == 2) {
+    if (excl) {
+      // (block-uniform) a pair with a rank-surface half stores its other half alone
 #pragma unroll
-    for (int jr = 0; jr < JR; jr++) {
-      const u32x2 lo = __builtin_bit_cast(u32x2, lds[rq[jr]]), hi = __builtin_bit_cast(u32x2, lds[rq[jr] + 1]);
-      const u32x4 x = {lo[0], lo[1], hi[0], hi[1]};
-      __builtin_amdgcn_raw_buffer_store_b128(x, rsrc, roff[jr], 0, 0);
+      for (int jr = 0; jr < JR; jr++) {
+        const u32x2 lo = __builtin_bit_cast(u32x2, lds[rq[jr]]), hi = __builtin_bit_cast(u32x2, lds[rq[jr] + 1]);
+        const bool used = roff[jr] != kOOB;
+        const bool m0 = used && marked(rq[jr]), m1 = used && marked(rq[jr] + 1);
+        if (m0 || m1) {
+          if (!m0) __builtin_amdgcn_raw_buffer_store_b64(lo, rsrc, roff[jr], 0, 0);
+          if (!m1) __builtin_amdgcn_raw_buffer_store_b64(hi, rsrc, roff[jr] + 8u, 0, 0);
+        } else {
+          const u32x4 x = {lo[0], lo[1], hi[0], hi[1]};
+          __builtin_amdgcn_raw_buffer_store_b128(x, rsrc, roff[jr], 0, 0);
+        }
+      }
+#pragma unroll
+      for (int jg = 0; jg < JG; jg++) {
+        const unsigned o = (goff[jg] != kOOB && marked(gq[jg])) ? kOOB : goff[jg];
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, lds[gq[jg]]), rsrc, o, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int jr = 0; jr < JR; jr++) {
+        const u32x2 lo = __builtin_bit_cast(u32x2, lds[rq[jr]]), hi = __builtin_bit_cast(u32x2, lds[rq[jr] + 1]);
+        const u32x4 x = {lo[0], lo[1], hi[0], hi[1]};
+        __builtin_amdgcn_raw_buffer_store_b128(x, rsrc, roff[jr], 0, 0);
+      }
+#pragma unroll
+      for (int jg = 0; jg < JG; jg++)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, lds[gq[jg]]), rsrc, goff[jg], 0, 0);
     }
+  }
+  if (excl) {
 #pragma unroll
-    for (int jg = 0; jg < JG; jg++)
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, lds[gq[jg]]), rsrc, goff[jg], 0, 0);
+    for (int j = (LW == 2 ? JO : 0); j < J; j++)
+      if (marked(j * kStreamThreads + tid)) keep &= ~(1u << j);
   }
 #pragma unroll
   for (int j = (LW == 2 ? JO : 0); j < J; j++) {
@@ -454,9 +520,10 @@ void k_flow(Geom G, FlowItems I)
   }
 }
 
-// Launches k_flow<n, S, lw, mode, tree> on nb blocks of kStreamThreads
-// (faces_flow.hip).  Tree 1 is compiled with load width 2 and modes 0 and 2.
+// Launches k_flow<n, S, lw, mode, tree> on nb item blocks of kStreamThreads
+// (faces_flow.hip), and for mode 1 the surface blocks W->nblk after them (W
+// may be null: none).  Tree 1 is compiled with load width 2 and modes 0 and 2.
 int launch_flow(int n, int S, int lw, int mode, int tree, const Geom &G, const FlowItems &F, unsigned nb,
-                hipStream_t st);
+                const SurfWork *W, hipStream_t st);
 
 }  // namespace faces

@@ -559,43 +559,66 @@ __host__ __device__ inline int surf_value(int k, int G)
   return k == 0 ? 0 : G - 1;
 }
 
+// Surface node t of the enumeration above: its copies (and received copies)
+// summed and written back.
+template <int N>
+__device__ __forceinline__ void surface_node(const Geom &G, const Bound &B, const double *__restrict__ recv,
+                                             const Slots &sl, const SurfDesc &S, long t)
+{
+  int gx, gy, gz;
+  if (t < S.cz) {
+    const long pl = (long)S.Gx * S.Gy;
+    const int k = (int)(t / pl);
+    const long r = t - k * pl;
+    gz = surf_value(k, S.Gz);
+    gy = (int)(r / S.Gx);
+    gx = (int)(r - (long)gy * S.Gx);
+  } else if (t < S.cz + S.cy) {
+    const long tt = t - S.cz;
+    const int niz = S.Gz - S.nsz;
+    const long pl = (long)S.Gx * niz;
+    const int k = (int)(tt / pl);
+    const long r = tt - k * pl;
+    gy = surf_value(k, S.Gy);
+    gz = 1 + (int)(r / S.Gx);
+    gx = (int)(r - (long)(gz - 1) * S.Gx);
+  } else {
+    const long tt = t - S.cz - S.cy;
+    const int niy = S.Gy - S.nsy, niz = S.Gz - S.nsz;
+    const long pl = (long)niy * niz;
+    const int k = (int)(tt / pl);
+    const long r = tt - k * pl;
+    gx = surf_value(k, S.Gx);
+    gz = 1 + (int)(r / niy);
+    gy = 1 + (int)(r - (long)(gz - 1) * niy);
+  }
+  dss_node_general<N>(G, B, recv, sl, gx, gy, gz);
+}
+
 template <int N>
 __global__ __launch_bounds__(256) void k_surface(Geom G, Bound B, const double *__restrict__ recv,
                                                  Slots sl, SurfDesc S)
 {
   const long total = S.cz + S.cy + S.cx;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
-       t += (long)gridDim.x * blockDim.x) {
-    int gx, gy, gz;
-    if (t < S.cz) {
-      const long pl = (long)S.Gx * S.Gy;
-      const int k = (int)(t / pl);
-      const long r = t - k * pl;
-      gz = surf_value(k, S.Gz);
-      gy = (int)(r / S.Gx);
-      gx = (int)(r - (long)gy * S.Gx);
-    } else if (t < S.cz + S.cy) {
-      const long tt = t - S.cz;
-      const int niz = S.Gz - S.nsz;
-      const long pl = (long)S.Gx * niz;
-      const int k = (int)(tt / pl);
-      const long r = tt - k * pl;
-      gy = surf_value(k, S.Gy);
-      gz = 1 + (int)(r / S.Gx);
-      gx = (int)(r - (long)(gz - 1) * S.Gx);
-    } else {
-      const long tt = t - S.cz - S.cy;
-      const int niy = S.Gy - S.nsy, niz = S.Gz - S.nsz;
-      const long pl = (long)niy * niz;
-      const int k = (int)(tt / pl);
-      const long r = tt - k * pl;
-      gx = surf_value(k, S.Gx);
-      gz = 1 + (int)(r / niy);
-      gy = 1 + (int)(r - (long)(gz - 1) * niy);
-    }
-    dss_node_general<N>(G, B, recv, sl, gx, gy, gz);
-  }
+       t += (long)gridDim.x * blockDim.x)
+    surface_node<N>(G, B, recv, sl, S, t);
 }
+
+// The surface sums as extra blocks of another launch (mode 1 of the one-item
+// kernel, whose item blocks leave every rank-surface location alone): blocks
+// [sfirst, sfirst + nblk) walk the surface nodes, the item blocks start at
+// block item0.
+struct SurfWork {
+  Bound B;
+  const double *recv;
+  Slots sl;
+  SurfDesc S;
+  unsigned nblk;    // surface blocks
+  unsigned sfirst;  // first surface block (set at launch)
+  unsigned item0;   // first item block (set at launch)
+  int lead;         // surface blocks before the items (1) or after them (0)
+};
 
 // ---------------------------------------------------------------------------
 // pack: the 26 messages (gpu/v-26-messages/Faces.cc:243-293)

@@ -113,7 +113,10 @@ struct faces_plan {
   int items_bpc;     // work-list item kernel: blocks per CU, a chunk of the list each
   int stream_planes; // 1: per-plane descriptors even below 4 GB
   int flow;          // periodic steps with compute = 1: 0 the streaming kernel; 1 the one-item
-                     // kernel (faces_flow.hpp), 8-byte lanes; 2 the same with 16-byte OWN loads
+                     // kernel (faces_flow.hpp), 8-byte lanes; 2 the same with 16-byte OWN
+                     // loads; 3 the owned runs in 16-byte pairs
+  int surface_blocks; // staged step on the one-item kernel: surface sums as blocks before (1)
+                      // or after (2) its items, or a launch of their own (0)
   // message arena: send slots then receive slots
   double *arena;
   long arena_doubles;
@@ -338,7 +341,8 @@ bool stream_ok(const faces_plan *p)
 }
 
 template <int N, int S>
-int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, hipStream_t st)
+int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, const SurfWork *W,
+                     hipStream_t st)
 {
   if (p->flow && !stream_big(p)) {
     // one item per block, items in memory order (faces_flow.hpp)
@@ -366,9 +370,10 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, hip
     const int lw = (p->flow == 3 || tree)                                                  ? 2
                    : (p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0) ? 1
                                                                                    : 0;
-    return launch_flow(N, S, lw, mode, tree, G, F, nb, st);
+    return launch_flow(N, S, lw, mode, tree, G, F, nb, W, st);
   }
-  if (tree) return fail(FACES_ERR_ARG, "the z, y, x tree is the one-item kernel's only");
+  if (tree || W)
+    return fail(FACES_ERR_ARG, "the z, y, x tree and the surface blocks are the one-item kernel's only");
   StreamItems I = {};
   I.my = p->my;
   I.mz = p->mz;
@@ -439,53 +444,55 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, hip
 }
 
 template <int N>
-int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, int tree, hipStream_t st)
+int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, int tree, const SurfWork *W,
+                    hipStream_t st)
 {
   switch (S) {
-    case 1: return launch_stream_ns<N, 1>(p, G, mode, tree, st);
-    case 2: return launch_stream_ns<N, 2>(p, G, mode, tree, st);
-    case 3: return launch_stream_ns<N, 3>(p, G, mode, tree, st);
-    case 4: return launch_stream_ns<N, 4>(p, G, mode, tree, st);
+    case 1: return launch_stream_ns<N, 1>(p, G, mode, tree, W, st);
+    case 2: return launch_stream_ns<N, 2>(p, G, mode, tree, W, st);
+    case 3: return launch_stream_ns<N, 3>(p, G, mode, tree, W, st);
+    case 4: return launch_stream_ns<N, 4>(p, G, mode, tree, W, st);
     default: break;
   }
   if constexpr (N >= 5 && N <= 8) {
-    if (S == 5) return launch_stream_ns<N, 5>(p, G, mode, tree, st);
+    if (S == 5) return launch_stream_ns<N, 5>(p, G, mode, tree, W, st);
   }
   if constexpr (N == 5) {
-    if (S == 10) return launch_stream_ns<N, 10>(p, G, mode, tree, st);
+    if (S == 10) return launch_stream_ns<N, 10>(p, G, mode, tree, W, st);
   }
   if constexpr (N <= 7) {
-    if (S == 6) return launch_stream_ns<N, 6>(p, G, mode, tree, st);
+    if (S == 6) return launch_stream_ns<N, 6>(p, G, mode, tree, W, st);
   }
   if constexpr (N <= 6) {
-    if (S == 8) return launch_stream_ns<N, 8>(p, G, mode, tree, st);
+    if (S == 8) return launch_stream_ns<N, 8>(p, G, mode, tree, W, st);
   }
   if constexpr (N <= 4) {
-    if (S == 16) return launch_stream_ns<N, 16>(p, G, mode, tree, st);
+    if (S == 16) return launch_stream_ns<N, 16>(p, G, mode, tree, W, st);
   }
   if constexpr (N == 2) {
-    if (S == 32) return launch_stream_ns<N, 32>(p, G, mode, tree, st);
+    if (S == 32) return launch_stream_ns<N, 32>(p, G, mode, tree, W, st);
   }
   return fail(FACES_ERR_ARG, "streaming kernel has no n=%d S=%d specialisation", N, S);
 }
 
-int launch_stream(const faces_plan *p, double *u, int mode, hipStream_t st, int tree = 0)
+int launch_stream(const faces_plan *p, double *u, int mode, hipStream_t st, int tree = 0,
+                  const SurfWork *W = nullptr)
 {
   const Geom G = make_geom(p, u);
   const int S = (p->flow && !stream_big(p)) ? flow_seg(p) : stream_seg(p);
   if (S == 0) return fail(FACES_ERR_ARG, "no streaming item length for n=%d mx=%d", p->n, p->mx);
   switch (p->n) {
-    case 2: return launch_stream_n<2>(p, G, S, mode, tree, st);
-    case 3: return launch_stream_n<3>(p, G, S, mode, tree, st);
-    case 4: return launch_stream_n<4>(p, G, S, mode, tree, st);
-    case 5: return launch_stream_n<5>(p, G, S, mode, tree, st);
-    case 6: return launch_stream_n<6>(p, G, S, mode, tree, st);
-    case 7: return launch_stream_n<7>(p, G, S, mode, tree, st);
-    case 8: return launch_stream_n<8>(p, G, S, mode, tree, st);
-    case 9: return launch_stream_n<9>(p, G, S, mode, tree, st);
-    case 10: return launch_stream_n<10>(p, G, S, mode, tree, st);
-    case 11: return launch_stream_n<11>(p, G, S, mode, tree, st);
-    case 12: return launch_stream_n<12>(p, G, S, mode, tree, st);
+    case 2: return launch_stream_n<2>(p, G, S, mode, tree, W, st);
+    case 3: return launch_stream_n<3>(p, G, S, mode, tree, W, st);
+    case 4: return launch_stream_n<4>(p, G, S, mode, tree, W, st);
+    case 5: return launch_stream_n<5>(p, G, S, mode, tree, W, st);
+    case 6: return launch_stream_n<6>(p, G, S, mode, tree, W, st);
+    case 7: return launch_stream_n<7>(p, G, S, mode, tree, W, st);
+    case 8: return launch_stream_n<8>(p, G, S, mode, tree, W, st);
+    case 9: return launch_stream_n<9>(p, G, S, mode, tree, W, st);
+    case 10: return launch_stream_n<10>(p, G, S, mode, tree, W, st);
+    case 11: return launch_stream_n<11>(p, G, S, mode, tree, W, st);
+    case 12: return launch_stream_n<12>(p, G, S, mode, tree, W, st);
     default: return fail(FACES_ERR_ARG, "streaming kernel has no n=%d specialisation", p->n);
   }
 }
@@ -549,6 +556,33 @@ int launch_box(const faces_plan *p, double *u, bool skip_surface, hipStream_t st
 #undef FACES_LAUNCH_BOX
   FACES_HIP(hipGetLastError());
   return FACES_OK;
+}
+
+// Whether the interior sums (launch_box with the surface skipped) run on the
+// one-item kernel, whose items leave every rank-surface location alone (mode
+// 1 marks those copies and stores none of them): its launch can take the
+// surface sums as extra blocks.
+bool interior_takes_surface(const faces_plan *p)
+{
+  int kern = active_kernel(p);
+  if (kern != FACES_VARIANT_STREAM)
+    kern = (p->variant == FACES_VARIANT_AUTO && stream_ok(p)) ? FACES_VARIANT_STREAM : FACES_VARIANT_DIRECT;
+  return p->surface_blocks && kern == FACES_VARIANT_STREAM && p->flow && !stream_big(p) && flow_seg(p) > 0;
+}
+
+// The interior sums and, as extra blocks of the same launch, the surface
+// sums of the staged step (interior_takes_surface(p) must hold).
+int launch_inner_surface(const faces_plan *p, double *u, const Bound &b, hipStream_t st)
+{
+  SurfWork W = {};
+  W.B = b;
+  W.recv = p->arena;
+  W.sl = p->recv;
+  W.S = surf_desc(p, b);
+  const long total = W.S.cz + W.S.cy + W.S.cx;
+  W.nblk = total > 0 ? (unsigned)grid_for(total, kStreamThreads) : 0u;
+  W.lead = p->surface_blocks == 1;
+  return launch_stream(p, u, 1, st, 0, &W);
 }
 
 int launch_surface(const faces_plan *p, double *u, const Bound &b, hipStream_t st)
@@ -752,6 +786,9 @@ int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, i
   // against 422); for even n, whose last-element rows end in singles, it
   // loads on 16-byte boundaries (flow 2; 40^3 n=12 363 us against 416).
   p->flow = (n % 2) ? 3 : 2;
+  // staged step, 60^3 n=7, interior rank: surface blocks after the items
+  // 259 us, before them 261, a launch of their own 268
+  p->surface_blocks = 2;
 
   // Message sizes, gpu/v-26-messages/Faces.cc:44-47, 64-66.
   const long nn = (long)n * n;
@@ -920,6 +957,10 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       if (value < 0 || value > 3) return fail(FACES_ERR_ARG, "one-item kernel mode %d outside [0,3]", value);
       p->flow = value;
       return FACES_OK;
+    case FACES_OPT_SURFACE_BLOCKS:
+      if (value < 0 || value > 2) return fail(FACES_ERR_ARG, "surface block placement %d outside [0,2]", value);
+      p->surface_blocks = value;
+      return FACES_OK;
     default: return fail(FACES_ERR_ARG, "unknown option %d", option);
   }
 }
@@ -939,6 +980,7 @@ int faces_plan_get_option(const faces_plan *p, int option, int *value)
     case FACES_OPT_STREAM_PLANES: *value = stream_big(p) ? 1 : 0; break;
     case FACES_OPT_ACTIVE_KERNEL: *value = active_kernel(p); break;
     case FACES_OPT_FLOW: *value = p->flow; break;
+    case FACES_OPT_SURFACE_BLOCKS: *value = p->surface_blocks; break;
     default: return fail(FACES_ERR_ARG, "unknown option %d", option);
   }
   return FACES_OK;
@@ -1152,13 +1194,15 @@ int faces_share(faces_plan *p, double *u, int compute, void *stream)
   // Staged: pack -> (exchange) -> inner -> unpack.  Inner and unpack touch
   // disjoint node sets, and pack reads only rank-surface copies, which inner
   // never writes (gpu/v-26-messages/Faces.cc:297-380 runs inner concurrently
-  // with pack for the same reason).
+  // with pack for the same reason).  On the one-item kernel, inner and unpack
+  // are one launch: its surface blocks run beside its item blocks.
   st = launch_pack(p, u, path == FACES_PATH_STAGED_DIRECT, s);
   if (st) return st;
   if (path == FACES_PATH_STAGED) {
     st = launch_exchange(p, s);
     if (st) return st;
   }
+  if (compute && interior_takes_surface(p)) return launch_inner_surface(p, u, bound_recv(p->mask), s);
   if (compute) {
     st = launch_box(p, u, true, s);
     if (st) return st;
@@ -1198,6 +1242,7 @@ int share_staged_on_arena(double *u, int rank, int lx, int ly, int lz, int mx, i
   p.arena = arena;
   st = launch_pack(&p, u, true, s);
   if (st) return st;
+  if (compute && interior_takes_surface(&p)) return launch_inner_surface(&p, u, bound_recv(p.mask), s);
   if (compute) {
     st = launch_box(&p, u, true, s);
     if (st) return st;

@@ -183,7 +183,12 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *                            same kernel loading and
  *                            storing exactly the locations each item owns, in
  *                            16-byte pairs that may start on any double; 1 =
- *                            8-byte lanes; 0 = the STREAM walks above */
+ *                            8-byte lanes; 0 = the STREAM walks above
+ *   FACES_OPT_SURFACE_BLOCKS the staged step's surface sums where its interior
+ *                            sums take the one-item kernel: 2 (default) =
+ *                            extra blocks of that launch placed after its
+ *                            items, 1 = before them, 0 = a launch of their
+ *                            own after it */
 #define FACES_OPT_VARIANT 0
 #define FACES_OPT_DIRECT_BLOCK 1
 #define FACES_OPT_SEG 2
@@ -195,6 +200,7 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
 #define FACES_OPT_STREAM_ROWS 8
 #define FACES_OPT_STREAM_PLANES 9
 #define FACES_OPT_FLOW 10
+#define FACES_OPT_SURFACE_BLOCKS 11
 int faces_plan_set_option(faces_plan *plan, int option, int value);
 int faces_plan_get_option(const faces_plan *plan, int option, int *value);
 

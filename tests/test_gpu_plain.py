@@ -426,6 +426,59 @@ def test_rank_positions_open_flow(torch_cuda, flow, xcd, seg):
             assert_same(got, want, shape)
 
 
+@pytest.mark.parametrize("path", [fa.PATH_STAGED, fa.PATH_STAGED_DIRECT])
+@pytest.mark.parametrize("rank", [13, 0, 22])
+@pytest.mark.parametrize("sblocks", [1, 2, 0])
+def test_staged_surface_blocks_bitexact(torch_cuda, path, rank, sblocks):
+    """The staged step's surface sums as blocks of the interior launch (before
+    or after its items, running beside them) or as a launch of their own: on
+    a field large enough for the blocks to overlap, three steps in a row stay
+    bit-equal to the oracle's staged form (random values)."""
+    shape = (7, 40, 30, 20)
+    n, mx, my, mz = shape
+    print("case staged surface blocks", path, rank, sblocks, flush=True)
+    mask = brute.mask_for_rank(rank, 3, 3, 3)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=300 + rank)
+    d = to_dev(torch_cuda, u)
+    f = fa.Faces(rank, 3, 3, 3, mx, my, mz, n, boundary=fa.OPEN)
+    f.set_path(path)
+    f.set_option(fa.OPT_SURFACE_BLOCKS, sblocks)
+    assert f.get_option(fa.OPT_SURFACE_BLOCKS) == sblocks
+    for _ in range(3):
+        f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    want = u.copy()
+    for _ in range(3):
+        oracle.share_staged(want, *shape, nbr_mask=mask)
+    assert_same(got, want, shape)
+
+
+@pytest.mark.parametrize("flow", [1, 2, 3])
+@pytest.mark.parametrize("sblocks", [1, 2])
+def test_staged_surface_blocks_all_ranks(torch_cuda, flow, sblocks):
+    """Every rank position of a 3x3x3 grid, staged direct, the surface sums
+    as blocks of the interior launch at each load width: bit-equal to the
+    oracle."""
+    shape = (5, 10, 4, 3)
+    n, mx, my, mz = shape
+    print("case staged surface blocks ranks", flow, sblocks, flush=True)
+    for rank in range(27):
+        mask = brute.mask_for_rank(rank, 3, 3, 3)
+        u = oracle.new_field(*shape, fa.FILL_RAND, seed=400 + rank)
+        d = to_dev(torch_cuda, u)
+        f = fa.Faces(rank, 3, 3, 3, mx, my, mz, n, boundary=fa.OPEN)
+        f.set_path(fa.PATH_STAGED_DIRECT)
+        f.set_option(fa.OPT_FLOW, flow)
+        f.set_option(fa.OPT_SURFACE_BLOCKS, sblocks)
+        f.share(d)
+        got = to_host(torch_cuda, d)
+        f.close()
+        want = u.copy()
+        oracle.share_staged(want, *shape, nbr_mask=mask)
+        assert_same(got, want, shape)
+
+
 def test_neighbor_table(torch_cuda):
     f = fa.Faces(13, 3, 3, 3, 2, 2, 2, 4, boundary=fa.OPEN)  # centre rank
     assert all(f.neighbor(k) >= 0 for k in range(26))
@@ -553,7 +606,8 @@ def test_invalid_arguments(torch_cuda):
     for bad in (-1, -5, 33):
         with pytest.raises(fa.FacesError):
             f.set_option(fa.OPT_STREAM_BPC, bad)
-    for opt, bad in ((fa.OPT_FLOW, 4), (fa.OPT_FLOW, -1), (fa.OPT_XCD, 3), (fa.OPT_STREAM_WALK, 4)):
+    for opt, bad in ((fa.OPT_FLOW, 4), (fa.OPT_FLOW, -1), (fa.OPT_XCD, 3), (fa.OPT_STREAM_WALK, 4),
+                     (fa.OPT_SURFACE_BLOCKS, 3), (fa.OPT_SURFACE_BLOCKS, -1)):
         with pytest.raises(fa.FacesError):
             f.set_option(opt, bad)
     f.close()
