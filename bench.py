@@ -84,6 +84,7 @@ def parse():
     ap.add_argument("--serve-bpc", type=int, default=0,
                     help="served leg: blocks per CU of batched launches (-1: one per item; 0: default)")
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--served-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -348,10 +349,28 @@ def load_traffic(name, plan_opts):
             "commit": d.get("measured_at_commit"), "source": d.get("source")}
 
 
+def served_in_child(args):
+    """The served leg in a child process of its own.  HIP shares a process's
+    GPU_MAX_HW_QUEUES hardware queues (4 by default) among its streams; the
+    streams the other legs leave behind (a graph-capture stream among them)
+    would put the server's two launch slots on one queue and serialise them
+    (p99 197-224 us against 161-167 us on queues of their own)."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--served-child", "--requests", str(args.requests),
+           "--streams", str(args.streams), "--serve-bpc", str(args.serve_bpc)]
+    out = subprocess.run(cmd, capture_output=True, text=True, check=True, timeout=600)
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
 def main():
     args = parse()
     if args.cpu_child:
         cpu_child(args.cpu_child)
+        return
+    if args.served_child:
+        import torch
+        import faces_amd as fa
+        print(json.dumps(served_leg(torch, fa, args.requests, args.streams, args.serve_bpc)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
@@ -443,7 +462,7 @@ def main():
         cpu = cpu_baseline(n, mx, my, mz, kind, args.cpu_seconds, periodic)
 
     batched = None if args.no_batch else batched_leg(torch, fa, args.grids, 50, 5, args.batch_tuning)
-    served = None if args.no_serve else served_leg(torch, fa, args.requests, args.streams, args.serve_bpc)
+    served = None if args.no_serve else served_in_child(args)
 
     per_sorted = sorted(per)
     rec = {

@@ -51,7 +51,8 @@ constexpr double kLargeBytes = 64.0 * 1024 * 1024;
 // Default launch shape of a batched launch, with two launches in flight: one
 // block per work item (Poisson 20k req/s: p50 32 us, p99 203 us), against
 // chunks of the launch's work list with 1, 2 or 4 blocks per CU (p50 39, 35,
-// 42 us; p99 220, 219, 228 us).
+// 42 us; p99 220, 219, 228 us), measured with the two slots' streams sharing
+// a hardware queue.
 constexpr int kServeBlocksPerCu = kBatchOneItem;
 
 double now_us(Clock::time_point t0) { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
