@@ -57,6 +57,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--path", type=int, default=0, help="FACES_PATH_* (0 auto)")
+    ap.add_argument("--rank-grid", default="1,1,1",
+                    help="open boundary: LX,LY,LZ ranks, the plain leg stepping one of them (--rank), "
+                         "its neighbours stood in for by self halos")
+    ap.add_argument("--rank", type=int, default=0, help="the rank the plain leg steps (--rank-grid)")
     ap.add_argument("--boundary", default="periodic", choices=["periodic", "open"],
                     help="plain leg on one rank: periodic (default), or open (the reference's own mode)")
     ap.add_argument("--block", type=int, default=0)
@@ -389,7 +393,11 @@ def main():
     for x in us:
         fa.fill(x, n, mx, my, mz, kind=kind, seed=42)
     periodic = args.boundary == "periodic"
-    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC if periodic else fa.OPEN)
+    lgrid = tuple(int(v) for v in args.rank_grid.split(","))
+    if len(lgrid) != 3 or (periodic and lgrid != (1, 1, 1)):
+        print("bench.py: --rank-grid takes LX,LY,LZ and needs --boundary open", file=sys.stderr)
+        sys.exit(2)
+    f = fa.Faces(args.rank, *lgrid, mx, my, mz, n, boundary=fa.PERIODIC if periodic else fa.OPEN)
     f.set_path(args.path)
     if args.block or args.seg >= 0 or args.xcd >= 0:
         f.set_tuning(args.block or 256, max(args.seg, 0), 1 if args.xcd < 0 else args.xcd)
@@ -447,7 +455,9 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         args.steps = nrep * per_rep
-    traffic = load_traffic(args.config if periodic else args.config + "_open", plan_opts)
+    # measured traffic exists for the default path of one rank only
+    traffic = (load_traffic(args.config if periodic else args.config + "_open", plan_opts)
+               if args.path == 0 and lgrid == (1, 1, 1) else None)
     per = [evs[i].elapsed_time(evs[i + 1]) / per_rep for i in range(nrep)]
     ms = sum(per) / len(per)
     gbs = b_step / (ms * 1e-3) / 1e9
@@ -481,7 +491,7 @@ def main():
         "config": {"workload": desc if periodic else
                    desc.replace("periodic", "open (the reference's own mode)", 1),
                    "n": n, "mx": mx, "my": my, "mz": mz,
-                   "rank_grid": f"1x1x1 {args.boundary}", "bytes_per_step": int(b_step),
+                   "rank_grid": f"{'x'.join(map(str, lgrid))} {args.boundary}" + (f", rank {args.rank}" if lgrid != (1, 1, 1) else ""), "bytes_per_step": int(b_step),
                    "fields_rotated": nbuf,
                    "cache_resident": bool(nbuf * 8 * total < 512e6),
                    "path": "fused" if args.path in (0, 1) else f"path{args.path}",
