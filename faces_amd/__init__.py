@@ -94,6 +94,7 @@ _sig("faces_pack", _c_int, _vp, _vp, _vp)
 _sig("faces_exchange", _c_int, _vp, _vp)
 _sig("faces_inner", _c_int, _vp, _vp, _vp)
 _sig("faces_unpack", _c_int, _vp, _vp, _vp)
+_sig("faces_inner_unpack", _c_int, _vp, _vp, _vp)
 _sig("faces_fill", _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _c_long, _c_int, _c_u64, _vp)
 _sig("faces_field_doubles", _c_long, _c_int, _c_int, _c_int, _c_int, _c_long)
 _sig("faces_stream_ceiling", _c_int, _vp, _c_long, _c_int, _vp, ctypes.POINTER(ctypes.c_double))
@@ -260,6 +261,11 @@ class Faces:
 
     def unpack(self, u, stream=None):
         _check(_lib.faces_unpack(self._h, _addr(u, self.doubles), _stream(stream)), "faces_unpack")
+
+    def inner_unpack(self, u, stream=None):
+        """inner and unpack in one launch, once the messages have arrived."""
+        _check(_lib.faces_inner_unpack(self._h, _addr(u, self.doubles), _stream(stream)),
+               "faces_inner_unpack")
 
 
 def header_symbols() -> list[str]:

@@ -20,8 +20,10 @@ int launch_flow_ns(int lw, int mode, int tree, const Geom &G, const FlowItems &F
                      (mode == 1 ? (size_t)kFlowMarkWords<N, S> * sizeof(unsigned) : 0);
   SurfWork sw = {};
   unsigned nblk = 0;
+  if (mode == 3 && (!W || W->nblk || lw != 2 || tree))
+    return fail(FACES_ERR_ARG, "one-item kernel: mode 3 takes the receive slots, load width 2, no surface blocks");
   if (W) {
-    if (mode != 1) return fail(FACES_ERR_ARG, "one-item kernel: surface blocks need mode 1");
+    if (mode != 1 && mode != 3) return fail(FACES_ERR_ARG, "one-item kernel: surface blocks need mode 1");
     sw = *W;
     // leading surface blocks: a multiple of 8, so that item block b still
     // runs on XCD b % 8
@@ -40,6 +42,8 @@ int launch_flow_ns(int lw, int mode, int tree, const Geom &G, const FlowItems &F
       return fail(FACES_ERR_ARG, "one-item kernel: tree order z,y,x needs load width 2 and mode 0 or 2");
     if (mode == 2) FACES_LAUNCH_FLOW(2, 2, 1);
     else FACES_LAUNCH_FLOW(2, 0, 1);
+  } else if (mode == 3) {
+    FACES_LAUNCH_FLOW(2, 3, 0);
   } else if (mode == 2) {
     if (lw == 2) FACES_LAUNCH_FLOW(2, 2, 0);
     else if (lw == 1) FACES_LAUNCH_FLOW(1, 2, 0);

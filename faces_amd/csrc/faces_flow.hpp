@@ -44,6 +44,13 @@ struct FlowItems {
 // rank: each side of the rank surface is a node of its own, which sums its
 // copies on that side, and those across the surface where the rank has a
 // neighbour there (stood in for by the self halo), x pairs, then y, then z.
+// MODE 3: the staged step's interior and surface sums in one pass: as MODE
+// 2, except that the copies across the rank surface are the received ones
+// (the message arena W.recv, its receive slots W.sl), not the field's own
+// copies on the other side: each side of the surface sums its local copies
+// and, where the rank has a neighbour on that side, the copies that
+// neighbour sent, in the tree order of the surface sums (faces_kernels.hpp,
+// dss_node_general).  Compiled with LW 2.
 // LW (load and store widths of the OWN region; the partner regions are moved
 // 8 bytes per lane):
 //   0  8 bytes per lane, loads and stores.
@@ -470,6 +477,98 @@ void k_flow(Geom G, FlowItems I, const SurfWork W)
       }
     }
   }
+  // MODE 3: the rank-surface nodes of this item, each side of the surface a
+  // node of its own that sums its local copies and the received ones
+  if constexpr (MODE == 3) {
+    if (surf) {
+#pragma unroll
+      for (int i = 0; i < NN; i++) {
+        const unsigned f = nflag[i];
+        if (!(f & 8u)) continue;
+        const unsigned wrap = surf & ((f & 6u) | ((f & 17u) == 17u ? 1u : 0u));
+        if (!wrap) continue;
+        // the node's element and local indices (the node table's enumeration)
+        const int kk = i * kStreamThreads + tid;
+        const int e = kk / SH::nshared;
+        int r = kk - e * SH::nshared;
+        int ix, iy, iz;
+        if (r < K * K) {
+          iz = 0; iy = r / K; ix = r - (r / K) * K;
+        } else if (r < K * K + K * (K - 1)) {
+          r -= K * K;
+          iz = 1 + r / K; iy = 0; ix = r - (r / K) * K;
+        } else {
+          r -= K * K + K * (K - 1);
+          constexpr int K1 = (K > 1) ? K - 1 : 1;
+          iz = 1 + r / K1; iy = 1 + r - (r / K1) * K1; ix = 0;
+        }
+        const unsigned pp[8] = {np[i][0] & 0xffff, np[i][0] >> 16, np[i][1] & 0xffff, np[i][1] >> 16,
+                                np[i][2] & 0xffff, np[i][2] >> 16, np[i][3] & 0xffff, np[i][3] >> 16};
+        const unsigned fc = f & 7u;
+        // own copy and partner copy (bit set) per axis: local index, element
+        const int jx = jx0 + e;
+        const int oi[3] = {ix, iy, iz}, oj[3] = {jx, jy, jz};
+        const int pj[3] = {e > 0 ? jx - 1 : jxs, jym, jzm};
+        const int m3[3] = {G.mx, G.my, G.mz};
+#pragma unroll
+        for (int sd = 0; sd < 8; sd++) {
+          if (sd & ~(int)wrap) continue;  // one node per side of each split axis
+          // pairs this side sums: inside the rank, or across the surface
+          // with a neighbour on that side
+          bool pa[3];
+#pragma unroll
+          for (int a = 0; a < 3; a++)
+            pa[a] = (fc & (1u << a)) && (!(wrap & (1u << a)) || (I.nbr_mask & (1 << (2 * a + ((sd >> a) & 1)))));
+          // leaves: bit a of l set = the partner copy on axis a (the received
+          // copy on a split axis)
+          double lv[8];
+#pragma unroll
+          for (int l = 0; l < 8; l++) {
+            lv[l] = 0.0;
+            if ((l & 1) && !pa[0]) continue;
+            if ((l & 2) && !pa[1]) continue;
+            if ((l & 4) && !pa[2]) continue;
+            const int dm = l & (int)wrap;  // received on these axes
+            if (!dm) {
+              lv[l] = lds[pp[l | sd]];
+              continue;
+            }
+            int ci[3], cj[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+              if (wrap & (1u << a)) {  // the side's own surface copy
+                const bool hi = (sd >> a) & 1;
+                ci[a] = hi ? K : 0;
+                cj[a] = hi ? m3[a] - 1 : 0;
+              } else {
+                const bool pt = (l >> a) & 1;
+                ci[a] = pt ? K : oi[a];
+                cj[a] = pt ? pj[a] : oj[a];
+              }
+            }
+            lv[l] = W.recv[recv_index(W.sl, n, G.mx, G.my, dm & 1, (dm >> 1) & 1, (dm >> 2) & 1, sd & 1,
+                                      (sd >> 1) & 1, (sd >> 2) & 1, ci[0], ci[1], ci[2], cj[0], cj[1], cj[2])];
+          }
+          // x pairs, then y, then z
+          double q[2];
+#pragma unroll
+          for (int lz = 0; lz < 2; lz++) {
+            const double p0 = pa[0] ? lv[4 * lz] + lv[4 * lz + 1] : lv[4 * lz];
+            const double p1 = pa[0] ? lv[4 * lz + 2] + lv[4 * lz + 3] : lv[4 * lz + 2];
+            q[lz] = pa[1] ? p0 + p1 : p0;
+          }
+          const double res = pa[2] ? q[0] + q[1] : q[0];
+          // every local copy of this side
+#pragma unroll
+          for (int nw = 0; nw < 8; nw++) {
+            if (nw & (int)wrap) continue;
+            if (nw & ~(int)fc) continue;
+            lds[pp[nw | sd]] = res;
+          }
+        }
+      }
+    }
+  }
   __syncthreads();
   // ---- stores of the locations this item owns; the others go out of range
   // (MODE 1: the copies of rank-surface nodes too)
@@ -522,7 +621,8 @@ void k_flow(Geom G, FlowItems I, const SurfWork W)
 
 // Launches k_flow<n, S, lw, mode, tree> on nb item blocks of kStreamThreads
 // (faces_flow.hip), and for mode 1 the surface blocks W->nblk after them (W
-// may be null: none).  Tree 1 is compiled with load width 2 and modes 0 and 2.
+// may be null: none); mode 3 reads its receive slots from W.  Tree 1 is
+// compiled with load width 2 and modes 0 and 2, mode 3 with load width 2.
 int launch_flow(int n, int S, int lw, int mode, int tree, const Geom &G, const FlowItems &F, unsigned nb,
                 const SurfWork *W, hipStream_t st);
 

@@ -115,8 +115,8 @@ struct faces_plan {
   int flow;          // periodic steps with compute = 1: 0 the streaming kernel; 1 the one-item
                      // kernel (faces_flow.hpp), 8-byte lanes; 2 the same with 16-byte OWN
                      // loads; 3 the owned runs in 16-byte pairs
-  int surface_blocks; // staged step on the one-item kernel: surface sums as blocks before (1)
-                      // or after (2) its items, or a launch of their own (0)
+  int surface_blocks; // staged step on the one-item kernel: surface sums by its items (3), as
+                      // blocks before (1) or after (2) its items, or a launch of their own (0)
   // message arena: send slots then receive slots
   double *arena;
   long arena_doubles;
@@ -367,7 +367,7 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, con
     // 16-byte loads on 16-byte boundaries (a 16-byte aligned field; else 8
     // bytes), flow 1 8 bytes per lane; the tree in z, y, x order (the
     // 6-message schedule's) is compiled with the 16-byte pairs only
-    const int lw = (p->flow == 3 || tree)                                                  ? 2
+    const int lw = (p->flow == 3 || tree || mode == 3)                                     ? 2
                    : (p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0) ? 1
                                                                                    : 0;
     return launch_flow(N, S, lw, mode, tree, G, F, nb, W, st);
@@ -580,6 +580,7 @@ int launch_inner_surface(const faces_plan *p, double *u, const Bound &b, hipStre
   W.sl = p->recv;
   W.S = surf_desc(p, b);
   const long total = W.S.cz + W.S.cy + W.S.cx;
+  if (p->surface_blocks == 3) return launch_stream(p, u, 3, st, 0, &W);  // summed by the items
   W.nblk = total > 0 ? (unsigned)grid_for(total, kStreamThreads) : 0u;
   W.lead = p->surface_blocks == 1;
   return launch_stream(p, u, 1, st, 0, &W);
@@ -786,8 +787,10 @@ int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, i
   // against 422); for even n, whose last-element rows end in singles, it
   // loads on 16-byte boundaries (flow 2; 40^3 n=12 363 us against 416).
   p->flow = (n % 2) ? 3 : 2;
-  // staged step, 60^3 n=7, interior rank: surface blocks after the items
-  // 259 us, before them 261, a launch of their own 268
+  // staged step, 60^3 n=7, interior rank of 3x3x3: surface blocks after the
+  // items 259.7 us, before them 261, a launch of their own 267.5, summed by
+  // the items (their scattered loads of received copies wait in the surface
+  // items) 265.8; corner rank: 257.8, -, 265.4, 252.2
   p->surface_blocks = 2;
 
   // Message sizes, gpu/v-26-messages/Faces.cc:44-47, 64-66.
@@ -958,7 +961,7 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       p->flow = value;
       return FACES_OK;
     case FACES_OPT_SURFACE_BLOCKS:
-      if (value < 0 || value > 2) return fail(FACES_ERR_ARG, "surface block placement %d outside [0,2]", value);
+      if (value < 0 || value > 3) return fail(FACES_ERR_ARG, "surface sums placement %d outside [0,3]", value);
       p->surface_blocks = value;
       return FACES_OK;
     default: return fail(FACES_ERR_ARG, "unknown option %d", option);
@@ -1133,6 +1136,17 @@ int faces_unpack(faces_plan *p, double *u, void *stream)
   int st = check_plan(p, u);
   if (st) return st;
   return launch_surface(p, u, bound_recv(p->mask), (hipStream_t)stream);
+}
+
+int faces_inner_unpack(faces_plan *p, double *u, void *stream)
+{
+  int st = check_plan(p, u);
+  if (st) return st;
+  const hipStream_t s = (hipStream_t)stream;
+  if (interior_takes_surface(p)) return launch_inner_surface(p, u, bound_recv(p->mask), s);
+  st = launch_box(p, u, true, s);
+  if (st) return st;
+  return launch_surface(p, u, bound_recv(p->mask), s);
 }
 
 int faces_share(faces_plan *p, double *u, int compute, void *stream)

@@ -187,8 +187,10 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *   FACES_OPT_SURFACE_BLOCKS the staged step's surface sums where its interior
  *                            sums take the one-item kernel: 2 (default) =
  *                            extra blocks of that launch placed after its
- *                            items, 1 = before them, 0 = a launch of their
- *                            own after it */
+ *                            items, 1 = before them, 3 = summed by that
+ *                            launch's items from their local copies and the
+ *                            received messages, 0 = a launch of their own
+ *                            after it */
 #define FACES_OPT_VARIANT 0
 #define FACES_OPT_DIRECT_BLOCK 1
 #define FACES_OPT_SEG 2
@@ -234,6 +236,11 @@ int faces_pack(faces_plan *plan, const double *u, void *stream);
 int faces_exchange(faces_plan *plan, void *stream);
 int faces_inner(faces_plan *plan, double *u, void *stream);
 int faces_unpack(faces_plan *plan, double *u, void *stream);
+/* inner and unpack together once the messages have arrived (Faces.cc:297-380
+ * and 419-671, whose kernels the reference runs in one step): one launch of
+ * the one-item kernel where the plan's field takes it (FACES_OPT_SURFACE_BLOCKS
+ * picks how), else faces_inner then faces_unpack. */
+int faces_inner_unpack(faces_plan *plan, double *u, void *stream);
 
 /* ---- field helpers ------------------------------------------------------- */
 

@@ -428,7 +428,7 @@ def test_rank_positions_open_flow(torch_cuda, flow, xcd, seg):
 
 @pytest.mark.parametrize("path", [fa.PATH_STAGED, fa.PATH_STAGED_DIRECT])
 @pytest.mark.parametrize("rank", [13, 0, 22])
-@pytest.mark.parametrize("sblocks", [1, 2, 0])
+@pytest.mark.parametrize("sblocks", [3, 1, 2, 0])
 def test_staged_surface_blocks_bitexact(torch_cuda, path, rank, sblocks):
     """The staged step's surface sums as blocks of the interior launch (before
     or after its items, running beside them) or as a launch of their own: on
@@ -455,7 +455,7 @@ def test_staged_surface_blocks_bitexact(torch_cuda, path, rank, sblocks):
 
 
 @pytest.mark.parametrize("flow", [1, 2, 3])
-@pytest.mark.parametrize("sblocks", [1, 2])
+@pytest.mark.parametrize("sblocks", [3, 1, 2])
 def test_staged_surface_blocks_all_ranks(torch_cuda, flow, sblocks):
     """Every rank position of a 3x3x3 grid, staged direct, the surface sums
     as blocks of the interior launch at each load width: bit-equal to the
@@ -607,7 +607,7 @@ def test_invalid_arguments(torch_cuda):
         with pytest.raises(fa.FacesError):
             f.set_option(fa.OPT_STREAM_BPC, bad)
     for opt, bad in ((fa.OPT_FLOW, 4), (fa.OPT_FLOW, -1), (fa.OPT_XCD, 3), (fa.OPT_STREAM_WALK, 4),
-                     (fa.OPT_SURFACE_BLOCKS, 3), (fa.OPT_SURFACE_BLOCKS, -1)):
+                     (fa.OPT_SURFACE_BLOCKS, 4), (fa.OPT_SURFACE_BLOCKS, -1)):
         with pytest.raises(fa.FacesError):
             f.set_option(opt, bad)
     f.close()

@@ -51,11 +51,16 @@ def whole_grid(fields, lgrid, shape):
     # same neighbour, and the per-direction slots keep the messages apart
     ((2, 1, 1), (5, 3, 2, 2), fa.PERIODIC), ((2, 2, 2), (4, 2, 2, 2), fa.PERIODIC),
     ((1, 3, 1), (6, 2, 2, 3), fa.PERIODIC)])
-def test_ranks_exchange_through_pieces(torch_cuda, lgrid, shape, boundary):
+@pytest.mark.parametrize("finish", ["inner+unpack", "inner_unpack", "inner_unpack blocks"])
+def test_ranks_exchange_through_pieces(torch_cuda, lgrid, shape, boundary, finish):
+    """After the exchange each rank finishes with faces_inner then
+    faces_unpack, or with faces_inner_unpack: one launch whose items sum the
+    rank surface from the messages other ranks sent (the default), or with
+    the surface sums as extra blocks of that launch."""
     n, mx, my, mz = shape
     lx, ly, lz = lgrid
     nr = lx * ly * lz
-    print("case transport", lgrid, shape, boundary, flush=True)
+    print("case transport", lgrid, shape, boundary, finish, flush=True)
     hip = hip_runtime()
     host = [oracle.new_field(*shape, fa.FILL_RAND, seed=50 + r) for r in range(nr)]
     dev = [torch_cuda.from_numpy(h.copy()).cuda() for h in host]
@@ -74,8 +79,13 @@ def test_ranks_exchange_through_pieces(torch_cuda, lgrid, shape, boundary):
             assert hip.hipMemcpy(dst, src, 8 * cnt, 3) == 0  # device to device
     torch_cuda.cuda.synchronize()
     for p, d in zip(plans, dev):
-        p.inner(d)
-        p.unpack(d)
+        if finish == "inner+unpack":
+            p.inner(d)
+            p.unpack(d)
+        else:
+            if finish.endswith("blocks"):
+                p.set_option(fa.OPT_SURFACE_BLOCKS, 2)
+            p.inner_unpack(d)
     torch_cuda.cuda.synchronize()
     got = [d.cpu().numpy() for d in dev]
     for p in plans:
