@@ -22,7 +22,8 @@ INCLUDE = os.path.join(ROOT, "include")
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["faces_plan.hip", "faces_flow.hip", "faces_batch.hip", "faces_serve.cpp", "faces_noalloc.hip"]
+HIP_SOURCES = ["faces_plan.hip", "faces_flow.hip", "faces_flow_g0.hip", "faces_flow_g1.hip", "faces_flow_g2.hip",
+               "faces_flow_g3.hip", "faces_batch.hip", "faces_serve.cpp", "faces_noalloc.hip"]
 OBJDIR = os.path.join(PKG, "_build")
 
 COMMON_FLAGS = [

@@ -196,18 +196,81 @@ void k_flow(Geom G, FlowItems I, const SurfWork W)
   constexpr int PRW = K / 2, PRW1 = PRW > 0 ? PRW : 1;    // pairs per row run of the last element
   constexpr int NPR = (S - 1) * PNL + K * K * PRW;
   constexpr int NSG = (K & 1) ? K * K : 0;                 // singles of the last element (n even)
-  constexpr int JR = (LW == 2 && NPR > 0) ? (NPR + kStreamThreads - 1) / kStreamThreads : 1;
-  constexpr int JG = (LW == 2 && NSG > 0) ? (NSG + kStreamThreads - 1) / kStreamThreads : 0;
+  // LW 3: the partner regions' owned runs in pairs too (z: an element's x-rows
+  // iy < n-1 of plane n-1; y: the x-row iy = n-1 of each plane iz < n-1; yz:
+  // the x-row iy = n-1 of plane n-1; the last element's runs end at ix = n-2),
+  // odd runs ending in a single, and the seam as singles
+  constexpr int ZPF = (K * n) / 2, ZPL = K / 2, ZPL1 = ZPL > 0 ? ZPL : 1;
+  constexpr int XPF = n / 2, XPL = K / 2, XPL1 = XPL > 0 ? XPL : 1;  // a run of one x-row
+  constexpr int NZP = (S - 1) * ZPF + K * ZPL;
+  constexpr int NYP = (S - 1) * K * XPF + K * XPL;
+  constexpr int NWP = (S - 1) * XPF + XPL;
+  constexpr int NZS = (K & 1) ? K : 0;                              // last element's z rows
+  constexpr int NYS = (S - 1) * K * (n & 1) + ((K & 1) ? K : 0);
+  constexpr int NWS = (S - 1) * (n & 1) + (K & 1);
+  constexpr int NPP = LW == 3 ? NZP + NYP + NWP : 0;
+  constexpr int NPS = LW == 3 ? NZS + NYS + NWS + SH::cS : 0;
+  constexpr int JR = (LW >= 2 && NPR + NPP > 0) ? (NPR + NPP + kStreamThreads - 1) / kStreamThreads : 1;
+  constexpr int JG = (LW >= 2 && NSG + NPS > 0) ? (NSG + NPS + kStreamThreads - 1) / kStreamThreads : 0;
   unsigned roff[JR], rq[JR];             // byte offset (kOOB: none), LDS position of the low half
   unsigned goff[JG > 0 ? JG : 1], gq[JG > 0 ? JG : 1];
   u32x4 wr[JR];
   double wg[JG > 0 ? JG : 1];
-  if constexpr (LW == 2) {
+  if constexpr (LW >= 2) {
 #pragma unroll
     for (int jr = 0; jr < JR; jr++) {
       const int t = jr * kStreamThreads + tid;
       roff[jr] = kOOB;
       rq[jr] = 0;
+      if (LW == 3 && t >= NPR && t < NPR + NPP) {  // a pair of a partner region
+        int tp = t - NPR;
+        int e, ro, lq, rbi;  // element, offset in the element, LDS position, row
+        if (tp < NZP) {
+          rbi = 2;
+          if (tp < (S - 1) * ZPF) {
+            e = tp / ZPF;
+            const int r = 2 * (tp - e * ZPF);
+            ro = K * n2 + r;
+            lq = SH::rZ + e * (K * n) + r;
+          } else {
+            const int t2 = tp - (S - 1) * ZPF, iy = t2 / ZPL1, ix = 2 * (t2 - iy * ZPL1);
+            e = S - 1;
+            ro = K * n2 + iy * n + ix;
+            lq = SH::rZ + e * (K * n) + iy * n + ix;
+          }
+        } else if ((tp -= NZP) < NYP) {
+          rbi = 1;
+          int iz, ix;
+          if (tp < (S - 1) * K * XPF) {
+            const int run = tp / XPF;
+            e = run / K;
+            iz = run - e * K;
+            ix = 2 * (tp - run * XPF);
+          } else {
+            const int t2 = tp - (S - 1) * K * XPF;
+            e = S - 1;
+            iz = t2 / XPL1;
+            ix = 2 * (t2 - iz * XPL1);
+          }
+          ro = iz * n2 + K * n + ix;
+          lq = SH::rY + e * (K * n) + iz * n + ix;
+        } else {
+          tp -= NYP;
+          rbi = 3;
+          int ix;
+          if (tp < (S - 1) * XPF) {
+            e = tp / XPF;
+            ix = 2 * (tp - e * XPF);
+          } else {
+            e = S - 1;
+            ix = 2 * (tp - (S - 1) * XPF);
+          }
+          ro = K * n2 + K * n + ix;
+          lq = SH::rW + e * n + ix;
+        }
+        roff[jr] = rb[rbi] + ((unsigned)(jx0 + e) * n3 + (unsigned)ro) * 8;
+        rq[jr] = (unsigned)lq;
+      }
       if (t < NPR) {
         int e, r0;
         if (t < (S - 1) * PNL) {
@@ -233,12 +296,49 @@ void k_flow(Geom G, FlowItems I, const SurfWork W)
         const int iz = t / K, iy = t - (t / K) * K, r = iz * n2 + iy * n + (K - 1);
         goff[jg] = rb[0] + ((unsigned)(jx0 + S - 1) * n3 + (unsigned)r) * 8;
         gq[jg] = (unsigned)((S - 1) * (K * n2) + r);
+      } else if (LW == 3 && t < NSG + NPS) {  // a single of a partner region, or the seam
+        int u = t - NSG;
+        if (u < NZS) {  // the last element's z rows (odd length)
+          const int ro = K * n2 + u * n + (K - 1);
+          goff[jg] = rb[2] + ((unsigned)(jx0 + S - 1) * n3 + (unsigned)ro) * 8;
+          gq[jg] = (unsigned)(SH::rZ + (S - 1) * (K * n) + u * n + (K - 1));
+        } else if ((u -= NZS) < NYS) {
+          int e, iz, ix;
+          if (u < (S - 1) * K * (n & 1)) {
+            e = u / K;
+            iz = u - e * K;
+            ix = n - 1;
+          } else {
+            e = S - 1;
+            iz = u - (S - 1) * K * (n & 1);
+            ix = K - 1;
+          }
+          goff[jg] = rb[1] + ((unsigned)(jx0 + e) * n3 + (unsigned)(iz * n2 + K * n + ix)) * 8;
+          gq[jg] = (unsigned)(SH::rY + e * (K * n) + iz * n + ix);
+        } else if ((u -= NYS) < NWS) {
+          const bool last = u >= (S - 1) * (n & 1);
+          const int e = last ? S - 1 : u, ix = last ? K - 1 : n - 1;
+          goff[jg] = rb[3] + ((unsigned)(jx0 + e) * n3 + (unsigned)(K * n2 + K * n + ix)) * 8;
+          gq[jg] = (unsigned)(SH::rW + e * n + ix);
+        } else {  // the seam: ix = n-1 of element jx0-1 in the four rows
+          const int q = u - NWS;
+          const unsigned sb = (unsigned)jxs * n3 * 8;
+          unsigned o;
+          if (q < K * K) o = rb[0] + sb + (unsigned)((q / K) * n2 + (q % K) * n + K) * 8;
+          else if (q < K * K + K) o = rb[1] + sb + (unsigned)((q - K * K) * n2 + K * n + K) * 8;
+          else if (q < K * K + 2 * K) o = rb[2] + sb + (unsigned)(K * n2 + (q - K * K - K) * n + K) * 8;
+          else o = rb[3] + sb + (unsigned)(K * n2 + K * n + K) * 8;
+          goff[jg] = o;
+          gq[jg] = (unsigned)(SH::rS + q);
+        }
       }
       wg[jg] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, goff[jg], 0, 0));
     }
+    if constexpr (LW == 2) {
 #pragma unroll
-    for (int j = JO; j < J; j++)
-      v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off[j], 0, 0));
+      for (int j = JO; j < J; j++)
+        v[j] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off[j], 0, 0));
+    }
   } else if constexpr (LW == 1) {
     const unsigned row_d = rb[0] / 8;  // row base in doubles
 #pragma unroll
@@ -316,7 +416,7 @@ void k_flow(Geom G, FlowItems I, const SurfWork W)
   const unsigned surf = MODE ? ((sg == 0 ? 1u : 0u) | (jy == 0 ? 2u : 0u) | (jz == 0 ? 4u : 0u)) : 0u;
 
   typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-  if constexpr (LW == 2) {
+  if constexpr (LW >= 2) {
 #pragma unroll
     for (int jr = 0; jr < JR; jr++) {
       if (roff[jr] == kOOB) continue;
@@ -327,8 +427,10 @@ void k_flow(Geom G, FlowItems I, const SurfWork W)
 #pragma unroll
     for (int jg = 0; jg < JG; jg++)
       if (goff[jg] != kOOB) lds[gq[jg]] = wg[jg];
+    if constexpr (LW == 2) {
 #pragma unroll
-    for (int j = JO; j < J; j++) lds[j * kStreamThreads + tid] = v[j];
+      for (int j = JO; j < J; j++) lds[j * kStreamThreads + tid] = v[j];
+    }
   } else if constexpr (LW == 1) {
 #pragma unroll
     for (int jw = 0; jw < JW; jw++) {
@@ -574,7 +676,11 @@ void k_flow(Geom G, FlowItems I, const SurfWork W)
   // (MODE 1: the copies of rank-surface nodes too)
   const bool excl = MODE == 1 && surf;
   auto marked = [&](unsigned pos) { return ((smark[pos >> 5] >> (pos & 31)) & 1u) != 0u; };
-  if constexpr (LW == 2) {
+  // the 8-byte slots stored below: OWN and partner regions (LW 0, 1), the
+  // partner regions (LW 2), none (LW 3: every owned location is in a pair or
+  // single slot)
+  constexpr int J8 = LW == 3 ? J : LW == 2 ? JO : 0;
+  if constexpr (LW >= 2) {
     if (excl) {
       // (block-uniform) a pair with a rank-surface half stores its other half alone
 #pragma unroll
@@ -609,11 +715,11 @@ void k_flow(Geom G, FlowItems I, const SurfWork W)
   }
   if (excl) {
 #pragma unroll
-    for (int j = (LW == 2 ? JO : 0); j < J; j++)
+    for (int j = J8; j < J; j++)
       if (marked(j * kStreamThreads + tid)) keep &= ~(1u << j);
   }
 #pragma unroll
-  for (int j = (LW == 2 ? JO : 0); j < J; j++) {
+  for (int j = J8; j < J; j++) {
     const u32x2 x = __builtin_bit_cast(u32x2, lds[j * kStreamThreads + tid]);
     __builtin_amdgcn_raw_buffer_store_b64(x, rsrc, (keep & (1u << j)) ? off[j] : kOOB, 0, 0);
   }

@@ -367,9 +367,13 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, con
     // 16-byte loads on 16-byte boundaries (a 16-byte aligned field; else 8
     // bytes), flow 1 8 bytes per lane; the tree in z, y, x order (the
     // 6-message schedule's) is compiled with the 16-byte pairs only
-    const int lw = (p->flow == 3 || tree || mode == 3)                                     ? 2
-                   : (p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0) ? 1
-                                                                                   : 0;
+    int lw = p->flow == 4                                                     ? 3
+             : p->flow == 3                                                   ? 2
+             : (p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0) ? 1
+                                                                             : 0;
+    // compiled with the owned runs in pairs: the 6-message tree, mode 3;
+    // the partner regions in pairs (flow 4): modes 0 and 2
+    if (tree || mode == 3 || (lw == 3 && mode == 1)) lw = 2;
     return launch_flow(N, S, lw, mode, tree, G, F, nb, W, st);
   }
   if (tree || W)
@@ -781,12 +785,16 @@ int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, i
   p->stream_planes = 0;
   // The one-item kernel for periodic steps and open ranks: 60^3 n=7 230-236
   // us against 274 us for the streaming kernel's column walk, 64^3 n=8 410
-  // against 449, 128^3 n=4 413-426 against 477.  For odd n it moves exactly
-  // the owned locations in 16-byte pairs (flow 3; runs of even length: 60^3
-  // n=7 229 us against 230-236, 48^3 n=7 116 against 123, 100^3 n=5 412
-  // against 422); for even n, whose last-element rows end in singles, it
-  // loads on 16-byte boundaries (flow 2; 40^3 n=12 363 us against 416).
-  p->flow = (n % 2) ? 3 : 2;
+  // against 449, 128^3 n=4 413-426 against 477.  Load width per n (us; flow
+  // 2: 16-byte loads on 16-byte boundaries, 3: the owned runs in 16-byte
+  // pairs, 4: the partner regions' runs in pairs too): 60^3 n=7 229.3 /
+  // 228.3 / 226.2; 80^3 n=6 356 / 366 / 349.6; 48^3 n=9 247 / 247 / 242;
+  // 40^3 n=11 280 / 275 / 267; 128^3 n=4 412 / 426 / 411; 100^3 n=5 413 /
+  // 413 / 412; 64^3 n=8 409 / 415 / 408; but 160^3 n=3 494 / 482 / 527,
+  // 44^3 n=10 266.5 / 281 / 272, 40^3 n=12 358 / 416 / 399, 256^3 n=2 728 /
+  // 745 / 763.
+  static const int kFlowDefault[13] = {0, 0, 2, 3, 4, 4, 4, 4, 4, 4, 2, 4, 2};
+  p->flow = (n >= 2 && n <= 12) ? kFlowDefault[n] : 2;
   // staged step, 60^3 n=7, interior rank of 3x3x3: surface blocks after the
   // items 259.7 us, before them 261, a launch of their own 267.5, summed by
   // the items (their scattered loads of received copies wait in the surface
@@ -957,7 +965,7 @@ int faces_plan_set_option(faces_plan *p, int option, int value)
       p->stream_planes = value ? 1 : 0;
       return FACES_OK;
     case FACES_OPT_FLOW:
-      if (value < 0 || value > 3) return fail(FACES_ERR_ARG, "one-item kernel mode %d outside [0,3]", value);
+      if (value < 0 || value > 4) return fail(FACES_ERR_ARG, "one-item kernel mode %d outside [0,4]", value);
       p->flow = value;
       return FACES_OK;
     case FACES_OPT_SURFACE_BLOCKS:

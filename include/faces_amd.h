@@ -175,15 +175,17 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *                            whether the plan addresses per plane
  *   FACES_OPT_FLOW           periodic steps with compute = 1, open ranks and
  *                            the staged step's interior sums (fields below
- *                            4 GB): 2 (default for even n) = the one-item
- *                            kernel, one
- *                            work item per block in memory order, OWN runs
- *                            loaded 16 bytes per lane (8 when the field is not
- *                            16-byte aligned); 3 (default for odd n) = the
- *                            same kernel loading and
- *                            storing exactly the locations each item owns, in
- *                            16-byte pairs that may start on any double; 1 =
- *                            8-byte lanes; 0 = the STREAM walks above
+ *                            4 GB): the one-item kernel, one work item per
+ *                            block in memory order, with load widths 2 = OWN
+ *                            runs loaded 16 bytes per lane (8 when the field
+ *                            is not 16-byte aligned; default for n = 2, 10,
+ *                            12); 3 = loading and storing exactly the
+ *                            locations each item owns, in 16-byte pairs that
+ *                            may start on any double (default for n = 3); 4
+ *                            = the same for the partner regions too (default
+ *                            for n = 4-9 and 11; the staged interior sums
+ *                            and the 6-message schedule take 3); 1 = 8-byte
+ *                            lanes; 0 = the STREAM walks above
  *   FACES_OPT_SURFACE_BLOCKS the staged step's surface sums where its interior
  *                            sums take the one-item kernel: 2 (default) =
  *                            extra blocks of that launch placed after its

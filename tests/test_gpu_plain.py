@@ -133,7 +133,7 @@ FLOW_SHAPES = [(7, 12, 6, 3), (7, 10, 3, 2), (8, 4, 3, 5), (5, 10, 2, 3), (5, 6,
                (12, 1, 2, 2), (7, 1, 1, 1), (7, 15, 9, 2)]
 
 
-@pytest.mark.parametrize("flow", [1, 2, 3])
+@pytest.mark.parametrize("flow", [1, 2, 3, 4])
 @pytest.mark.parametrize("xcd", [0, 1, 2])
 @pytest.mark.parametrize("shape", FLOW_SHAPES)
 @pytest.mark.parametrize("seg", [0, 1, 3, 5])
@@ -179,11 +179,12 @@ def test_flow_padded_planes(torch_cuda, flow, shape, extra):
     assert np.array_equal(got.view(np.uint64), run_oracle(u, shape, ps=ps, nshare=3).view(np.uint64))
 
 
-@pytest.mark.parametrize("flow", [2, 3])
+@pytest.mark.parametrize("flow", [2, 3, 4])
 def test_flow_misaligned_field(torch_cuda, flow):
     """A field that starts 8 bytes into its allocation: flow 2 does not take
-    its 16-byte aligned loads (it falls back to 8-byte lanes), flow 3 moves
-    its pairs from any double; results exact, the bytes around untouched."""
+    its 16-byte aligned loads (it falls back to 8-byte lanes), flows 3 and 4
+    move their pairs from any double; results exact, the bytes around
+    untouched."""
     shape = (7, 10, 4, 3)
     n, mx, my, mz = shape
     print("case flow misaligned", flow, shape, flush=True)
@@ -191,7 +192,7 @@ def test_flow_misaligned_field(torch_cuda, flow):
     buf = torch_cuda.zeros(u.size + 2, dtype=torch_cuda.float64, device="cuda")
     buf[1:1 + u.size] = torch_cuda.from_numpy(u).to("cuda")
     f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
-    assert f.get_option(fa.OPT_FLOW) == 3  # the default for odd n
+    assert f.get_option(fa.OPT_FLOW) == 4  # the default for n = 7
     f.set_option(fa.OPT_FLOW, flow)
     f.share(int(buf.data_ptr()) + 8)
     got = to_host(torch_cuda, buf)
@@ -374,7 +375,7 @@ def test_rank_positions_open_streaming(torch_cuda, walk, rows, bpc, seg, planes)
         assert_same(got, want, shape)
 
 
-@pytest.mark.parametrize("flow,xcd", [(2, 2), (1, 1), (2, 0), (3, 2), (3, 1)])
+@pytest.mark.parametrize("flow,xcd", [(2, 2), (1, 1), (2, 0), (3, 2), (3, 1), (4, 2), (4, 1)])
 @pytest.mark.parametrize("shape,seg", [((7, 12, 4, 3), 0), ((5, 10, 3, 2), 5), ((8, 4, 3, 5), 0),
                                        ((4, 16, 2, 3), 0), ((6, 5, 2, 2), 1), ((2, 4, 1, 3), 0),
                                        ((12, 2, 2, 2), 0)])
@@ -398,7 +399,7 @@ def test_open_rank_flow_bitexact(torch_cuda, flow, xcd, shape, seg):
     assert_same(got, run_oracle(u, shape, periodic=False, nshare=2), shape)
 
 
-@pytest.mark.parametrize("flow,xcd,seg", [(2, 2, 0), (1, 1, 3), (2, 0, 1), (3, 2, 0), (3, 1, 2)])
+@pytest.mark.parametrize("flow,xcd,seg", [(2, 2, 0), (1, 1, 3), (2, 0, 1), (3, 2, 0), (3, 1, 2), (4, 2, 0), (4, 1, 2)])
 def test_rank_positions_open_flow(torch_cuda, flow, xcd, seg):
     """Every rank position of a 3x3x3 open decomposition through the
     one-item kernel (neighbours as self halos): bit-equal to the staged
@@ -606,7 +607,7 @@ def test_invalid_arguments(torch_cuda):
     for bad in (-1, -5, 33):
         with pytest.raises(fa.FacesError):
             f.set_option(fa.OPT_STREAM_BPC, bad)
-    for opt, bad in ((fa.OPT_FLOW, 4), (fa.OPT_FLOW, -1), (fa.OPT_XCD, 3), (fa.OPT_STREAM_WALK, 4),
+    for opt, bad in ((fa.OPT_FLOW, 5), (fa.OPT_FLOW, -1), (fa.OPT_XCD, 3), (fa.OPT_STREAM_WALK, 4),
                      (fa.OPT_SURFACE_BLOCKS, 4), (fa.OPT_SURFACE_BLOCKS, -1)):
         with pytest.raises(fa.FacesError):
             f.set_option(opt, bad)
