@@ -42,9 +42,10 @@ int launch_flow_ns(int lw, int mode, int tree, const Geom &G, const FlowItems &F
     else FACES_LAUNCH_FLOW(2, 0, 1);
   } else if (mode == 3) {
     FACES_LAUNCH_FLOW(2, 3, 0);
-  } else if (lw == 3 && (mode == 0 || mode == 2)) {
+  } else if (lw == 3) {
     // the partner regions in 16-byte pairs too
     if (mode == 2) FACES_LAUNCH_FLOW(3, 2, 0);
+    else if (mode == 1) FACES_LAUNCH_FLOW(3, 1, 0);
     else FACES_LAUNCH_FLOW(3, 0, 0);
   } else if (mode == 2) {
     if (lw == 2) FACES_LAUNCH_FLOW(2, 2, 0);

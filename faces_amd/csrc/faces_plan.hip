@@ -372,8 +372,8 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, con
              : (p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0) ? 1
                                                                              : 0;
     // compiled with the owned runs in pairs: the 6-message tree, mode 3;
-    // the partner regions in pairs (flow 4): modes 0 and 2
-    if (tree || mode == 3 || (lw == 3 && mode == 1)) lw = 2;
+    // the partner regions in pairs (flow 4): modes 0, 1 and 2
+    if (tree || mode == 3) lw = 2;
     return launch_flow(N, S, lw, mode, tree, G, F, nb, W, st);
   }
   if (tree || W)
