@@ -728,7 +728,8 @@ void k_flow(Geom G, FlowItems I, const SurfWork W)
 // Launches k_flow<n, S, lw, mode, tree> on nb item blocks of kStreamThreads
 // (faces_flow.hip), and for mode 1 the surface blocks W->nblk after them (W
 // may be null: none); mode 3 reads its receive slots from W.  Tree 1 is
-// compiled with load width 2 and modes 0 and 2, mode 3 with load width 2.
+// compiled with load widths 2 and 3 and modes 0 and 2, mode 3 with load
+// width 2.
 int launch_flow(int n, int S, int lw, int mode, int tree, const Geom &G, const FlowItems &F, unsigned nb,
                 const SurfWork *W, hipStream_t st);
 

@@ -35,11 +35,17 @@ int launch_flow_ns(int lw, int mode, int tree, const Geom &G, const FlowItems &F
 #define FACES_LAUNCH_FLOW(LW, MODE, TREE) k_flow<N, S, LW, MODE, TREE><<<grid, blk, lds, st>>>(G, F, sw)
   if (tree) {
     // the 6-message schedule's tree: owned runs in 16-byte pairs (any
-    // alignment), periodic or open
-    if (lw != 2 || mode == 1)
-      return fail(FACES_ERR_ARG, "one-item kernel: tree order z,y,x needs load width 2 and mode 0 or 2");
-    if (mode == 2) FACES_LAUNCH_FLOW(2, 2, 1);
-    else FACES_LAUNCH_FLOW(2, 0, 1);
+    // alignment; load width 3: the partner regions' too), periodic or open
+    if (lw < 2 || mode == 1 || mode == 3)
+      return fail(FACES_ERR_ARG, "one-item kernel: tree order z,y,x needs load width 2 or 3 and mode 0 or 2");
+    if (lw == 3) {
+      if (mode == 2) FACES_LAUNCH_FLOW(3, 2, 1);
+      else FACES_LAUNCH_FLOW(3, 0, 1);
+    } else if (mode == 2) {
+      FACES_LAUNCH_FLOW(2, 2, 1);
+    } else {
+      FACES_LAUNCH_FLOW(2, 0, 1);
+    }
   } else if (mode == 3) {
     FACES_LAUNCH_FLOW(2, 3, 0);
   } else if (lw == 3) {

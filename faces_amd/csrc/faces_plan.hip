@@ -371,9 +371,9 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, con
              : p->flow == 3                                                   ? 2
              : (p->flow == 2 && reinterpret_cast<uintptr_t>(G.u) % 16 == 0) ? 1
                                                                              : 0;
-    // compiled with the owned runs in pairs: the 6-message tree, mode 3;
-    // the partner regions in pairs (flow 4): modes 0, 1 and 2
-    if (tree || mode == 3) lw = 2;
+    // the 6-message tree: the owned runs in pairs, the partner regions' too
+    // with flow 4; mode 3: the owned runs in pairs
+    if ((tree && lw < 2) || mode == 3) lw = 2;
     return launch_flow(N, S, lw, mode, tree, G, F, nb, W, st);
   }
   if (tree || W)

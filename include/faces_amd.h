@@ -183,9 +183,10 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *                            locations each item owns, in 16-byte pairs that
  *                            may start on any double (default for n = 3); 4
  *                            = the same for the partner regions too (default
- *                            for n = 4-9 and 11; the staged interior sums
- *                            and the 6-message schedule take 3); 1 = 8-byte
- *                            lanes; 0 = the STREAM walks above
+ *                            for n = 4-9 and 11; the staged step's surface
+ *                            sums done by its items, FACES_OPT_SURFACE_BLOCKS
+ *                            3, take 3); 1 = 8-byte lanes; 0 = the STREAM
+ *                            walks above
  *   FACES_OPT_SURFACE_BLOCKS the staged step's surface sums where its interior
  *                            sums take the one-item kernel: 2 (default) =
  *                            extra blocks of that launch placed after its
