@@ -172,8 +172,8 @@ def batched_leg(torch, fa, ngrids, steps, warmup, tuning=None):
     del fields
     torch.cuda.synchronize()
     return {"workload": f"{ngrids} periodic grids, mx,my,mz uniform [2,24], n from {{4,6,8}}, "
-                        "integer-valued [0,1023]; one work-list launch per step (a chunk per block) "
-                        "replayed from a HIP graph",
+                        "integer-valued [0,1023]; one work-list launch per step (launch form in "
+                        "'tuned': blocks_per_cu -1 is one item per block) replayed from a HIP graph",
             "grids": ngrids, "bytes_per_step": int(nbytes),
             "ms_per_step": round(ms_b, 5), "grids_per_s": round(ngrids / (ms_b * 1e-3), 1),
             "GBs": round(nbytes / (ms_b * 1e-3) / 1e9, 2),

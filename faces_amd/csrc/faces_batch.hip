@@ -20,7 +20,7 @@ namespace faces {
 // Static schedule, no atomics: the work list is cut into one contiguous
 // chunk of equal estimated cost per block, the chunks of XCD x (blocks with
 // blockIdx % 8 == x, round-robin dispatch) forming a contiguous eighth of the
-// list.  A block's items of one grid are consecutive element segments and
+// list's cost.  A block's items of one grid are consecutive element segments and
 // rows: they form a run, summed with the next item's loads in flight, and the
 // neighbouring items whose sectors an item shares were summed just before by
 // the same block, so those lines are still in its L2.
@@ -128,7 +128,8 @@ __global__ __launch_bounds__(kStreamThreads, 4) void k_batch(BatchLaunch D)
 // the batched kernel, one item per block
 // ---------------------------------------------------------------------------
 // Every block takes one item of the work list; XCD x (block b runs on XCD
-// b % 8) takes the x-th contiguous eighth of the list, dispatched in order.
+// b % 8) takes its contiguous share of the list (shares of equal estimated
+// cost), dispatched in order; blocks past the share exit at once.
 // A grid's items are in memory order, so an item's seam, y and z partners
 // (one, nseg and my*nseg items back) were summed moments before on the same
 // XCD and their shared lines meet in its L2; with no loop, short-lived
@@ -158,9 +159,8 @@ void k_batch_flow(BatchLaunch D)
 {
   extern __shared__ double lds[];
   const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
-  const int q8 = D.nunits >> 3, rem8 = D.nunits & 7;
-  if (k >= q8 + (xcd < rem8 ? 1 : 0)) return;
-  const int u = xcd * q8 + (xcd < rem8 ? xcd : rem8) + k;
+  const int u = D.xs[xcd] + k;
+  if (u >= D.xs[xcd + 1]) return;
   // the grid holding item u: the number of grids whose first item is at or
   // before u, counted 64 grids per step across the wave's lanes (grids are
   // sorted by first item)
@@ -230,11 +230,15 @@ int batch_seg(int n, int mx, int cap)
   return best;
 }
 
-int batch_blocks(int nunits, int blocks_per_cu)
+int batch_blocks(const BatchLaunch &D, int blocks_per_cu)
 {
-  if (blocks_per_cu == kBatchOneItem) return kXcds * ((nunits + kXcds - 1) / kXcds);
+  if (blocks_per_cu == kBatchOneItem) {
+    int most = 0;
+    for (int x = 0; x < kXcds; x++) most = std::max(most, D.xs[x + 1] - D.xs[x]);
+    return kXcds * most;
+  }
   long nb = (long)device_cus() * blocks_per_cu;
-  nb = std::min(nb, (long)nunits + kXcds - 1);
+  nb = std::min(nb, (long)D.nunits + kXcds - 1);
   nb = (nb + kXcds - 1) / kXcds * kXcds;
   return (int)std::max((long)kXcds, nb);
 }
@@ -244,7 +248,11 @@ int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t s
   if (D.nunits == 0) return FACES_OK;
   if (nblocks % kXcds) return fail(FACES_ERR_ARG, "batched launch of %d blocks", nblocks);
   if (one_item) {
-    if (nblocks < D.nunits) return fail(FACES_ERR_ARG, "one-item launch of %d blocks for %d items", nblocks, D.nunits);
+    if (D.xs[0] != 0 || D.xs[kXcds] != D.nunits) return fail(FACES_ERR_ARG, "XCD shares do not cover the list");
+    for (int x = 0; x < kXcds; x++)
+      if (D.xs[x + 1] < D.xs[x] || (long)kXcds * (D.xs[x + 1] - D.xs[x]) > nblocks)
+        return fail(FACES_ERR_ARG, "one-item launch of %d blocks for an XCD share of %d items", nblocks,
+                    D.xs[x + 1] - D.xs[x]);
     k_batch_flow<<<dim3(nblocks), dim3(kStreamThreads), lds_bytes, st>>>(D);
     FACES_HIP(hipGetLastError());
     return FACES_OK;
@@ -254,11 +262,30 @@ int launch_batch(const BatchLaunch &D, int nblocks, int lds_bytes, hipStream_t s
   return FACES_OK;
 }
 
-// Estimated cost of one item of a grid, in doubles moved: the segment's
-// points plus a fixed per-item latency share.
+// Estimated cost of one item of a grid, in doubles moved: the mean segment's
+// points plus a fixed per-item share, the latency of a short block.  (256
+// mixed grids of n 4, 6 and 8, one item per block: share 0 598.8 us, 320
+// 589.9, 640 584.8, 1000 581.1, 1600 575.2, 2600 583.6.)
 static double item_cost(const BatchGrid &x)
 {
-  return (double)std::min(x.S, x.mx) * x.n * x.n * x.n + 320.0;
+  return (double)x.mx * x.n * x.n * x.n / x.nseg + 1600.0;
+}
+
+// XCD shares of equal cost: xs[x] is the first item whose cost prefix
+// reaches x/kXcds of the total.
+static void split_xcds(const std::vector<BatchGrid> &g, BatchLaunch *D)
+{
+  size_t i = 0;
+  D->xs[0] = 0;
+  for (int x = 1; x < kXcds; x++) {
+    const double t = D->total * x / kXcds;
+    while (i + 1 < g.size() && g[i].cost0 + g[i].items * g[i].icost < t) i++;
+    const BatchGrid &G = g[i];
+    const double k = std::ceil((t - G.cost0) / G.icost);
+    const int it = G.first + (k <= 0.0 ? 0 : k >= (double)G.items ? G.items : (int)k);
+    D->xs[x] = std::max(D->xs[x - 1], std::min(it, D->nunits));
+  }
+  D->xs[kXcds] = D->nunits;
 }
 
 int launch_single(double *u, int n, int mx, int my, int mz, long plane_stride, int seg_cap,
@@ -286,15 +313,15 @@ int launch_single(double *u, int n, int mx, int my, int mz, long plane_stride, i
   D.ngrids = 1;
   D.nunits = x.items;
   D.total = (double)x.items;
-  return launch_batch(D, batch_blocks(x.items, blocks_per_cu), batch_lds_bytes(n, x.S), st,
+  split_xcds(std::vector<BatchGrid>(1, x), &D);
+  return launch_batch(D, batch_blocks(D, blocks_per_cu), batch_lds_bytes(n, x.S), st,
                       blocks_per_cu == kBatchOneItem);
 }
 
-int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, double *total)
+int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, BatchLaunch *D)
 {
-  // Items of expensive grids first: the work list then ends on cheap items.
-  std::vector<int> order(g.size());
-  std::iota(order.begin(), order.end(), 0);
+  std::vector<int> by_cost(g.size());
+  std::iota(by_cost.begin(), by_cost.end(), 0);
   for (auto &x : g) {
     x.S = batch_seg(x.n, x.mx, cap);
     if (x.S == 0) return fail(FACES_ERR_ARG, "grid n=%d mx=%d has no batched item shape", x.n, x.mx);
@@ -302,7 +329,20 @@ int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, 
     x.items = x.my * x.mz * x.nseg;
     x.icost = item_cost(x);
   }
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return g[a].icost > g[b].icost; });
+  std::stable_sort(by_cost.begin(), by_cost.end(), [&](int a, int b) { return g[a].icost > g[b].icost; });
+  // Deal the grids, costliest items first, into one bin per XCD in snake
+  // order (0..7, 7..0, ...) and lay the bins end to end: each XCD's share of
+  // equal cost then holds the same mix of item sizes, and within it the
+  // items of expensive grids come first, so it ends on cheap items.  (Sorted
+  // list cut into eighths of equal item count: one XCD held 1.29 x the mean
+  // bytes of 256 mixed grids.)
+  std::vector<int> order;
+  order.reserve(g.size());
+  for (int bin = 0; bin < kXcds; bin++)
+    for (size_t i = 0; i < by_cost.size(); i++) {
+      const int k = (int)(i % (2 * kXcds));
+      if ((k < kXcds ? k : 2 * kXcds - 1 - k) == bin) order.push_back(by_cost[i]);
+    }
   std::vector<BatchGrid> sorted;
   int first = 0, lds = 0;
   double cost = 0.0;
@@ -317,8 +357,9 @@ int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, 
   }
   g.swap(sorted);
   *lds_bytes = lds;
-  *nunits = first;
-  *total = cost;
+  D->nunits = first;
+  D->total = cost;
+  split_xcds(g, D);
   return FACES_OK;
 }
 
@@ -333,10 +374,13 @@ struct faces_batch {
   std::vector<double *> u;
   std::vector<faces_plan *> plans;  // one-launch-per-grid baseline and fallback paths
   std::vector<int> solo;            // grids without a batched item shape (n > 12): own plan
-  // device state
+  // device state: the work list's descriptors and its launch (grids of
+  // every element size in one list: with XCD shares of equal cost one launch
+  // ran 256 mixed grids of n 4, 6 and 8 in 589.6 us, one launch per n in
+  // 605.1 us)
   BatchGrid *d_grids = nullptr;
-  double total = 0.0;
-  int lds_bytes = 0, nunits = 0, nbatched = 0;
+  BatchLaunch D = {};
+  int lds_bytes = 0;
   // tuning: one block per item (256 mixed grids, n 4-8, 2-24 elements a
   // side: 626 us, traffic 1.02 x the fields; chunks of the list with 32
   // blocks per CU 685 us, traffic 1.36 x)
@@ -379,18 +423,20 @@ int upload(faces_batch *b)
     x.ps = x.ys * x.my;
     x.field_bytes = (unsigned)(8 * x.ps * x.mz);
   }
+  b->D = BatchLaunch{};
   b->lds_bytes = 0;
-  b->nunits = 0;
   if (!g.empty()) {
-    int st = plan_grids(g, b->seg_cap, &b->lds_bytes, &b->nunits, &b->total);
+    int st = plan_grids(g, b->seg_cap, &b->lds_bytes, &b->D);
     if (st) return st;
+    b->D.grids = b->d_grids;
+    b->D.firsts = reinterpret_cast<const int *>(b->d_grids + g.size());
+    b->D.ngrids = (int)g.size();
     // launches of the old descriptors may still run on any stream
     FACES_HIP(hipDeviceSynchronize());
     std::vector<char> host(batch_desc_bytes((int)g.size()));
     const size_t bytes = batch_pack_desc(g, host.data());
     FACES_HIP(hipMemcpy(b->d_grids, host.data(), bytes, hipMemcpyHostToDevice));
   }
-  b->nbatched = (int)g.size();
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
@@ -403,18 +449,10 @@ int upload(faces_batch *b)
   return FACES_OK;
 }
 
-BatchLaunch launch_desc(const faces_batch *b)
+int launch_list(const faces_batch *b, hipStream_t st)
 {
-  BatchLaunch D = {};
-  D.grids = b->d_grids;
-  D.firsts = reinterpret_cast<const int *>(b->d_grids + b->nbatched);
-  D.ngrids = b->nbatched;
-  D.nunits = b->nunits;
-  D.total = b->total;
-  return D;
+  return launch_batch(b->D, batch_blocks(b->D, b->bpc), b->lds_bytes, st, b->bpc == kBatchOneItem);
 }
-
-int nblocks_of(const faces_batch *b) { return batch_blocks(b->nunits, b->bpc); }
 
 int ensure_graph(faces_batch *b)
 {
@@ -422,9 +460,9 @@ int ensure_graph(faces_batch *b)
     int st = upload(b);
     if (st) return st;
   }
-  if (b->exec || b->nunits == 0) return FACES_OK;
+  if (b->exec || b->D.nunits == 0) return FACES_OK;
   FACES_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-  int st = launch_batch(launch_desc(b), nblocks_of(b), b->lds_bytes, b->cap_stream, b->bpc == kBatchOneItem);
+  int st = launch_list(b, b->cap_stream);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
   if (st) {
@@ -600,12 +638,10 @@ int faces_batch_tune(faces_batch *b, int reps, void *stream, double *out)
       b->dirty = true;
       rc = upload(b);
       if (rc) break;
-      const BatchLaunch D = launch_desc(b);
-      const int nb = nblocks_of(b);
-      rc = launch_batch(D, nb, b->lds_bytes, st, bpc == kBatchOneItem);  // warm-up
+      rc = launch_list(b, st);  // warm-up
       for (int r = 0; r < reps && !rc; r++) {
         if (r == 0 && hipEventRecord(e0, st) != hipSuccess) rc = fail(FACES_ERR_HIP, "event");
-        if (!rc) rc = launch_batch(D, nb, b->lds_bytes, st, bpc == kBatchOneItem);
+        if (!rc) rc = launch_list(b, st);
       }
       if (rc) break;
       float ms = 0;

@@ -10,10 +10,10 @@
 
 namespace faces {
 
-// One batched launch: grids sorted by first item.  The work list is cut into
-// one contiguous chunk of equal estimated cost per block; chunks are ordered
-// XCD by XCD (the hardware dispatches block b to XCD b % 8), so each XCD
-// holds a contiguous eighth of the list.
+// One batched launch: grids sorted by first item.  The hardware dispatches
+// block b to XCD b % 8; each XCD takes a contiguous share of the list of
+// equal estimated cost (xs), as one item per block or, in the chunked form,
+// as chunks of equal cost.
 constexpr int kXcds = 8;
 struct BatchLaunch {
   const BatchGrid *grids;  // null: the single grid g0 (passed by value)
@@ -21,6 +21,7 @@ struct BatchLaunch {
   int ngrids;
   int nunits;
   double total;            // estimated cost of the whole list
+  int xs[kXcds + 1];       // first item of each XCD's share; xs[kXcds] = nunits
   BatchGrid g0;
 };
 
@@ -37,17 +38,20 @@ size_t batch_pack_desc(const std::vector<BatchGrid> &g, void *buf);
 int batch_seg(int n, int mx, int cap);
 int batch_lds_bytes(int n, int S);
 
-// Fills S/nseg/items, sorts the grids by item cost and assigns first-item
-// offsets and cost prefixes; returns the LDS bytes, item count and total
-// estimated cost of the launch.
-int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, int *nunits, double *total);
+// Fills S/nseg/items, orders the grids (dealt by item cost into one bin per
+// XCD, so every XCD's share holds the same mix of item sizes) and assigns
+// first-item offsets and cost prefixes; returns the LDS bytes and fills D's
+// item count, total estimated cost and XCD shares (the caller sets grids,
+// firsts and ngrids).
+int plan_grids(std::vector<BatchGrid> &g, int cap, int *lds_bytes, BatchLaunch *D);
 
 // blocks_per_cu value that selects the one-item-per-block kernel.
 constexpr int kBatchOneItem = -1;
 
 // Blocks of a launch: blocks_per_cu per CU (a chunk of the list each), or,
-// for kBatchOneItem, one per item; a multiple of kXcds.
-int batch_blocks(int nunits, int blocks_per_cu);
+// for kBatchOneItem, kXcds times the largest XCD share (one block per item;
+// the blocks past an XCD's share exit at once); a multiple of kXcds.
+int batch_blocks(const BatchLaunch &D, int blocks_per_cu);
 
 // Enqueues the batched kernel: nblocks blocks (a multiple of kXcds) taking a
 // chunk of the list each, or (one_item) one item each.

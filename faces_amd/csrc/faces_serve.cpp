@@ -152,19 +152,16 @@ int faces_server::launch(Slot &s, std::vector<Request *> &batch)
     }
   }
   if (!g.empty()) {
-    int lds = 0, nunits = 0;
-    double total = 0.0;
-    int st = plan_grids(g, 64, &lds, &nunits, &total);
+    int lds = 0;
+    BatchLaunch D = {};
+    int st = plan_grids(g, 64, &lds, &D);
     if (st) return st;
     const size_t bytes = batch_pack_desc(g, s.h_grids);
     FACES_HIP(hipMemcpyAsync(s.d_grids, s.h_grids, bytes, hipMemcpyHostToDevice, s.stream));
-    BatchLaunch D = {};
     D.grids = s.d_grids;
     D.firsts = reinterpret_cast<const int *>(s.d_grids + g.size());
     D.ngrids = (int)g.size();
-    D.nunits = nunits;
-    D.total = total;
-    st = launch_batch(D, batch_blocks(nunits, bpc), lds, s.stream, bpc == kBatchOneItem);
+    st = launch_batch(D, batch_blocks(D, bpc), lds, s.stream, bpc == kBatchOneItem);
     if (st) return st;
   }
   for (Request *r : other) {
