@@ -214,6 +214,27 @@ def test_batch_with_solo_grids(torch_cuda):
     b.close()
 
 
+@pytest.mark.parametrize("shapes", [
+    [(4, 2, 1, 1)],                                     # one item: seven XCD shares empty
+    [(2, 1, 1, 1)] * 5 + [(3, 1, 2, 1)],                # fewer items than XCDs
+    [(8, 24, 24, 24)] + [(2, 1, 1, 1)] * 40,            # one grid holds most of the cost
+    [(2, 1, 1, 1)] * 40 + [(12, 3, 2, 2), (5, 17, 1, 9)],
+])
+@pytest.mark.parametrize("bpc", [-1, 4])
+def test_batch_uneven_xcd_shares(torch_cuda, shapes, bpc):
+    """Work lists whose XCD shares of equal cost hold very different item
+    counts (or none): every item is summed once."""
+    print("case batch uneven shares", len(shapes), bpc, flush=True)
+    fields = make_fields(torch_cuda, shapes, seed0=300)
+    b = Batch(shapes)
+    b.set_fields(fields)
+    b.set_tuning(256, 0, bpc)
+    b.share()
+    b.share()
+    assert_all_same(host(torch_cuda, fields), oracle_steps(shapes, fa.FILL_RAND, 300, 2), shapes)
+    b.close()
+
+
 def test_batch_rebind_fields(torch_cuda):
     """New field pointers re-upload the descriptors and re-capture the graph."""
     print("case batch rebind", flush=True)
