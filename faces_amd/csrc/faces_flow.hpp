@@ -27,7 +27,8 @@ namespace faces {
 
 struct FlowItems {
   int my, mz;
-  int nseg;              // mx / S
+  int nseg;              // items per element row: the launch covers columns [jxb, jxb + nseg*S)
+  int jxb;               // first element column of the launch's items
   int nitems;            // my*mz*nseg
   unsigned field_bytes;  // below kMaxBufferBytes: offsets are 32-bit
   int nbr_mask;          // MODE 2: face neighbours of the rank, bit 2a (2a+1) for the low (high)
@@ -127,7 +128,7 @@ void k_flow(Geom G, FlowItems I, const SurfWork W)
   const int jy = row % G.my, jz = row / G.my;
   const int jym = (jy == 0) ? G.my - 1 : jy - 1;
   const int jzm = (jz == 0) ? G.mz - 1 : jz - 1;
-  const int jx0 = sg * S;
+  const int jx0 = I.jxb + sg * S;
   const int jxs = (jx0 == 0) ? G.mx - 1 : jx0 - 1;
   // byte offsets of the four rows: own, y partner, z partner, yz partner
   const unsigned rb[4] = {(unsigned)((jz * G.ps + jy * G.ys) * 8), (unsigned)((jz * G.ps + jym * G.ys) * 8),
@@ -413,7 +414,7 @@ void k_flow(Geom G, FlowItems I, const SurfWork W)
   }
   // MODE 1, 2: axes on which this item holds the rank surface (its partners
   // across it are the wrap partners: segment 0, row jy = 0, plane jz = 0)
-  const unsigned surf = MODE ? ((sg == 0 ? 1u : 0u) | (jy == 0 ? 2u : 0u) | (jz == 0 ? 4u : 0u)) : 0u;
+  const unsigned surf = MODE ? ((jx0 == 0 ? 1u : 0u) | (jy == 0 ? 2u : 0u) | (jz == 0 ? 4u : 0u)) : 0u;
 
   typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
   if constexpr (LW >= 2) {

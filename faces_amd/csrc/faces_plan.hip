@@ -312,19 +312,45 @@ int stream_seg(const faces_plan *p)
   return 0;
 }
 
-// Elements per item of the one-item kernel: as stream_seg, with its own
-// defaults per n (60^3 n=7: 5 elements 249-253 us, 4 elements 254-256 us,
-// 6 elements 251-252 us).
+// The largest compiled item length up to `want` elements that keeps the
+// per-thread tables in registers, fits `cols` columns and (divides) divides
+// them; 0 if none.
+static int item_len(int n, int want, int cols, bool divides)
+{
+  for (int s = std::min(std::min(want, 32), cols); s >= 1; s--) {
+    if (!stream_case(n, s) || (divides && cols % s)) continue;
+    if (stream_slots(n, s) <= kStreamMaxSlots) return s;
+  }
+  return 0;
+}
+
+// Elements per item of the one-item kernel, with its own defaults per n
+// (60^3 n=7: 5 elements 249-253 us, 4 elements 254-256 us, 6 elements
+// 251-252 us).  Where the longest item dividing mx is shorter than the
+// longest that fits by 3 or more elements (61 columns, n=7: 1 against 5),
+// items of the longer length cover the first mx - mx % S columns and a
+// second launch of items of flow_rest elements the rest: each element owns
+// the nodes on its low faces whatever item it is in, so items of different
+// lengths sum disjoint nodes.  (n=7, 60x60 rows and planes: 61 columns 243
+// us split, 320 us on the work-list kernel; 62 columns 250 us split, 376 us
+// with 2-element items; 63 columns 257 us split, 249 us with 3-element
+// items.)
 int flow_seg(const faces_plan *p)
 {
   if (p->n < 2 || p->n > 12) return 0;
   static const int kDefault[13] = {0, 0, 32, 16, 16, 10, 5, 5, 4, 3, 2, 1, 1};
   const int want = p->seg_len > 0 ? p->seg_len : kDefault[p->n];
-  for (int s = std::min(want, 32); s >= 1; s--) {
-    if (!stream_case(p->n, s) || p->mx % s) continue;
-    if (stream_slots(p->n, s) <= kStreamMaxSlots) return s;
-  }
-  return 0;
+  const int sd = item_len(p->n, want, p->mx, true), sw = item_len(p->n, want, p->mx, false);
+  return sw >= sd + 3 ? sw : sd;
+}
+
+// Elements per item of the second launch over the last mx % flow_seg
+// columns; 0 if there are none.
+int flow_rest(const faces_plan *p)
+{
+  const int S = flow_seg(p);
+  if (S == 0 || p->mx % S == 0) return 0;
+  return item_len(p->n, S, p->mx % S, true);
 }
 
 // Byte offsets of the streaming kernel are 32-bit: within the field, or,
@@ -340,16 +366,19 @@ bool stream_ok(const faces_plan *p)
   return stream_seg(p) > 0 && plane < kMaxBufferBytes;
 }
 
+// Items of S elements over the element columns [jxb, jxb + cols) (the
+// streaming kernel: all columns).
 template <int N, int S>
 int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, const SurfWork *W,
-                     hipStream_t st)
+                     hipStream_t st, int jxb, int cols)
 {
   if (p->flow && !stream_big(p)) {
     // one item per block, items in memory order (faces_flow.hpp)
     FlowItems F;
     F.my = p->my;
     F.mz = p->mz;
-    F.nseg = p->mx / S;
+    F.nseg = cols / S;
+    F.jxb = jxb;
     F.nitems = p->my * p->mz * F.nseg;
     F.field_bytes = (unsigned)(8 * faces_field_doubles(p->n, p->mx, p->my, p->mz, p->ps));
     F.xcd_remap = p->xcd_remap;
@@ -378,6 +407,7 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, con
   }
   if (tree || W)
     return fail(FACES_ERR_ARG, "the z, y, x tree and the surface blocks are the one-item kernel's only");
+  if (jxb != 0 || cols != p->mx) return fail(FACES_ERR_ARG, "the streaming kernel takes whole element rows");
   StreamItems I = {};
   I.my = p->my;
   I.mz = p->mz;
@@ -449,34 +479,51 @@ int launch_stream_ns(const faces_plan *p, const Geom &G, int mode, int tree, con
 
 template <int N>
 int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, int tree, const SurfWork *W,
-                    hipStream_t st)
+                    hipStream_t st, int jxb, int cols)
 {
   switch (S) {
-    case 1: return launch_stream_ns<N, 1>(p, G, mode, tree, W, st);
-    case 2: return launch_stream_ns<N, 2>(p, G, mode, tree, W, st);
-    case 3: return launch_stream_ns<N, 3>(p, G, mode, tree, W, st);
-    case 4: return launch_stream_ns<N, 4>(p, G, mode, tree, W, st);
+    case 1: return launch_stream_ns<N, 1>(p, G, mode, tree, W, st, jxb, cols);
+    case 2: return launch_stream_ns<N, 2>(p, G, mode, tree, W, st, jxb, cols);
+    case 3: return launch_stream_ns<N, 3>(p, G, mode, tree, W, st, jxb, cols);
+    case 4: return launch_stream_ns<N, 4>(p, G, mode, tree, W, st, jxb, cols);
     default: break;
   }
   if constexpr (N >= 5 && N <= 8) {
-    if (S == 5) return launch_stream_ns<N, 5>(p, G, mode, tree, W, st);
+    if (S == 5) return launch_stream_ns<N, 5>(p, G, mode, tree, W, st, jxb, cols);
   }
   if constexpr (N == 5) {
-    if (S == 10) return launch_stream_ns<N, 10>(p, G, mode, tree, W, st);
+    if (S == 10) return launch_stream_ns<N, 10>(p, G, mode, tree, W, st, jxb, cols);
   }
   if constexpr (N <= 7) {
-    if (S == 6) return launch_stream_ns<N, 6>(p, G, mode, tree, W, st);
+    if (S == 6) return launch_stream_ns<N, 6>(p, G, mode, tree, W, st, jxb, cols);
   }
   if constexpr (N <= 6) {
-    if (S == 8) return launch_stream_ns<N, 8>(p, G, mode, tree, W, st);
+    if (S == 8) return launch_stream_ns<N, 8>(p, G, mode, tree, W, st, jxb, cols);
   }
   if constexpr (N <= 4) {
-    if (S == 16) return launch_stream_ns<N, 16>(p, G, mode, tree, W, st);
+    if (S == 16) return launch_stream_ns<N, 16>(p, G, mode, tree, W, st, jxb, cols);
   }
   if constexpr (N == 2) {
-    if (S == 32) return launch_stream_ns<N, 32>(p, G, mode, tree, W, st);
+    if (S == 32) return launch_stream_ns<N, 32>(p, G, mode, tree, W, st, jxb, cols);
   }
   return fail(FACES_ERR_ARG, "streaming kernel has no n=%d S=%d specialisation", N, S);
+}
+
+template <int N>
+int launch_stream_cols(const faces_plan *p, const Geom &G, int S, int mode, int tree, const SurfWork *W,
+                       hipStream_t st)
+{
+  const bool flow = p->flow && !stream_big(p);
+  const int cols = flow ? p->mx - p->mx % S : p->mx;
+  int rc = launch_stream_n<N>(p, G, S, mode, tree, W, st, 0, cols);
+  if (rc || cols == p->mx) return rc;
+  // the last mx % S columns: items of their own length, no surface blocks
+  SurfWork w = {};
+  if (W) {
+    w = *W;
+    w.nblk = 0;
+  }
+  return launch_stream_n<N>(p, G, flow_rest(p), mode, tree, W ? &w : nullptr, st, cols, p->mx - cols);
 }
 
 int launch_stream(const faces_plan *p, double *u, int mode, hipStream_t st, int tree = 0,
@@ -485,18 +532,20 @@ int launch_stream(const faces_plan *p, double *u, int mode, hipStream_t st, int 
   const Geom G = make_geom(p, u);
   const int S = (p->flow && !stream_big(p)) ? flow_seg(p) : stream_seg(p);
   if (S == 0) return fail(FACES_ERR_ARG, "no streaming item length for n=%d mx=%d", p->n, p->mx);
+  if (p->flow && !stream_big(p) && p->mx % S && flow_rest(p) == 0)
+    return fail(FACES_ERR_ARG, "no item length for the last %d columns", p->mx % S);
   switch (p->n) {
-    case 2: return launch_stream_n<2>(p, G, S, mode, tree, W, st);
-    case 3: return launch_stream_n<3>(p, G, S, mode, tree, W, st);
-    case 4: return launch_stream_n<4>(p, G, S, mode, tree, W, st);
-    case 5: return launch_stream_n<5>(p, G, S, mode, tree, W, st);
-    case 6: return launch_stream_n<6>(p, G, S, mode, tree, W, st);
-    case 7: return launch_stream_n<7>(p, G, S, mode, tree, W, st);
-    case 8: return launch_stream_n<8>(p, G, S, mode, tree, W, st);
-    case 9: return launch_stream_n<9>(p, G, S, mode, tree, W, st);
-    case 10: return launch_stream_n<10>(p, G, S, mode, tree, W, st);
-    case 11: return launch_stream_n<11>(p, G, S, mode, tree, W, st);
-    case 12: return launch_stream_n<12>(p, G, S, mode, tree, W, st);
+    case 2: return launch_stream_cols<2>(p, G, S, mode, tree, W, st);
+    case 3: return launch_stream_cols<3>(p, G, S, mode, tree, W, st);
+    case 4: return launch_stream_cols<4>(p, G, S, mode, tree, W, st);
+    case 5: return launch_stream_cols<5>(p, G, S, mode, tree, W, st);
+    case 6: return launch_stream_cols<6>(p, G, S, mode, tree, W, st);
+    case 7: return launch_stream_cols<7>(p, G, S, mode, tree, W, st);
+    case 8: return launch_stream_cols<8>(p, G, S, mode, tree, W, st);
+    case 9: return launch_stream_cols<9>(p, G, S, mode, tree, W, st);
+    case 10: return launch_stream_cols<10>(p, G, S, mode, tree, W, st);
+    case 11: return launch_stream_cols<11>(p, G, S, mode, tree, W, st);
+    case 12: return launch_stream_cols<12>(p, G, S, mode, tree, W, st);
     default: return fail(FACES_ERR_ARG, "streaming kernel has no n=%d specialisation", p->n);
   }
 }

@@ -616,13 +616,17 @@ def test_invalid_arguments(torch_cuda):
 
 @pytest.mark.parametrize("shape,kernel", [((7, 60, 2, 2), fa.VARIANT_STREAM),
                                           ((7, 9, 2, 2), fa.VARIANT_STREAM),
-                                          ((7, 13, 3, 2), fa.VARIANT_ITEMS),
-                                          ((8, 7, 2, 3), fa.VARIANT_ITEMS),
-                                          ((4, 23, 2, 1), fa.VARIANT_ITEMS),
+                                          ((7, 13, 3, 2), fa.VARIANT_STREAM),
+                                          ((8, 7, 2, 3), fa.VARIANT_STREAM),
+                                          ((4, 23, 2, 1), fa.VARIANT_STREAM),
+                                          ((9, 7, 2, 2), fa.VARIANT_ITEMS),
                                           ((13, 2, 2, 2), fa.VARIANT_DIRECT)])
 def test_auto_kernel_choice_bitexact(torch_cuda, shape, kernel):
     """AUTO takes the work-list kernel (ragged rows) when its items are more
-    than twice as long as the streaming kernel's, else the streaming kernel."""
+    than twice as long as the one-item kernel's, else the one-item kernel,
+    whose rows the item length does not divide are split between two item
+    lengths (13, 7 and 23 columns above); n=9, 7 columns: one-element items
+    against the work-list kernel's 3."""
     n, mx, my, mz = shape
     print("case auto kernel", shape, kernel, flush=True)
     f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
@@ -836,3 +840,84 @@ def test_plan_tune_then_bitexact(torch_cuda, shape, boundary):
     got = to_host(torch_cuda, d)
     f.close()
     assert_same(got, run_oracle(u, shape, periodic=boundary == fa.PERIODIC), shape)
+
+
+# ------------------ rows the item length does not divide: two item launches
+
+# (n, mx, my, mz) where the longest item dividing mx is 3 or more elements
+# shorter than the default item: items of the default length cover the first
+# mx - mx % S columns, a second launch the rest (n=7 mx=11: 2 items of 5 and
+# one of 1 per row; mx=13: 2 of 5, one of 3; n=5 mx=23: 2 of 10, one of 3;
+# n=8 mx=7: one of 4, one of 3; n=6 mx=7: one of 5, one of 2; n=4 mx=17: one
+# of 16, one of 1; n=2 mx=35: one of 32, one of 3).
+SPLIT_SHAPES = [(7, 11, 3, 2), (7, 13, 2, 3), (5, 23, 2, 2), (8, 7, 3, 2), (6, 7, 2, 2), (4, 17, 2, 2),
+                (2, 35, 2, 1)]
+
+
+@pytest.mark.parametrize("flow", [1, 2, 3, 4])
+@pytest.mark.parametrize("xcd", [0, 1, 2])
+@pytest.mark.parametrize("shape", SPLIT_SHAPES)
+def test_flow_split_rows_bitexact(torch_cuda, flow, xcd, shape):
+    """Periodic step over rows split between two item lengths, at each load
+    width and XCD order: two shares of a random field, bit-exact."""
+    n, mx, my, mz = shape
+    print("case flow split rows", flow, xcd, shape, flush=True)
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=317)
+    d = to_dev(torch_cuda, u)
+    f = fa.Faces(0, 1, 1, 1, mx, my, mz, n, boundary=fa.PERIODIC)
+    f.set_option(fa.OPT_VARIANT, fa.VARIANT_STREAM)
+    f.set_option(fa.OPT_FLOW, flow)
+    f.set_option(fa.OPT_XCD, xcd)
+    assert f.get_option(fa.OPT_ACTIVE_KERNEL) == fa.VARIANT_STREAM
+    for _ in range(2):
+        f.share(d)
+    got = to_host(torch_cuda, d)
+    f.close()
+    assert_same(got, run_oracle(u, shape, nshare=2), shape)
+
+
+@pytest.mark.parametrize("path,sblocks", [(fa.PATH_FUSED, 2), (fa.PATH_STAGED_DIRECT, 2),
+                                          (fa.PATH_STAGED_DIRECT, 1), (fa.PATH_STAGED_DIRECT, 3),
+                                          (fa.PATH_STAGED_DIRECT, 0), (fa.PATH_STAGED, 2)])
+@pytest.mark.parametrize("shape", [(7, 11, 3, 2), (6, 7, 2, 2), (5, 23, 2, 2)])
+def test_split_rows_open_ranks(torch_cuda, path, sblocks, shape):
+    """Open ranks of a 3x3x3 grid over split rows: the fused sweep, and the
+    staged step with its surface sums as blocks of the first launch (after
+    or before its items), by the items from the received messages, or as a
+    launch of their own; bit-equal to the oracle's staged form."""
+    n, mx, my, mz = shape
+    print("case split rows open", path, sblocks, shape, flush=True)
+    for rank in (13, 0, 22, 5, 26):
+        mask = brute.mask_for_rank(rank, 3, 3, 3)
+        u = oracle.new_field(*shape, fa.FILL_RAND, seed=600 + rank)
+        d = to_dev(torch_cuda, u)
+        f = fa.Faces(rank, 3, 3, 3, mx, my, mz, n, boundary=fa.OPEN)
+        f.set_path(path)
+        f.set_option(fa.OPT_SURFACE_BLOCKS, sblocks)
+        for _ in range(2):
+            f.share(d)
+        got = to_host(torch_cuda, d)
+        f.close()
+        want = u.copy()
+        for _ in range(2):
+            oracle.share_staged(want, *shape, nbr_mask=mask)
+        assert_same(got, want, shape)
+
+
+@pytest.mark.parametrize("shape", [(7, 11, 3, 2), (8, 7, 3, 2)])
+def test_split_rows_six(torch_cuda, shape):
+    """The 6-message schedule in one launch per item length (tree z, y, x)
+    over split rows: open ranks and the periodic step, bit-exact."""
+    n, mx, my, mz = shape
+    u = oracle.new_field(*shape, fa.FILL_RAND, seed=505)
+    for rank in (13, 0, 26):
+        print("case split rows six", shape, rank, flush=True)
+        got = _six_gpu(torch_cuda, u, shape, fa.OPEN, rank, (3, 3, 3))
+        want = u.copy()
+        oracle.share_six(want, *shape, nbr_mask=brute.mask_for_rank(rank, 3, 3, 3))
+        assert_same(got, want, shape)
+    got = _six_gpu(torch_cuda, u, shape, fa.PERIODIC, 0, (1, 1, 1), nshare=2)
+    want = u.copy()
+    for _ in range(2):
+        oracle.share_six(want, *shape, nbr_mask=63)
+    assert_same(got, want, shape)
