@@ -804,8 +804,10 @@ int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, i
   // XCD order: each XCD takes an eighth of every plane's items (the one-item
   // kernel; 64^3 n=8 410 us against 423 for contiguous eighths of the list,
   // 80^3 n=6 356 against 397); the other kernels read it as 1, contiguous
-  // eighths of their work list.
-  p->xcd_remap = 2;
+  // eighths of their work list.  n=2 (64-byte elements, 8-byte lanes):
+  // contiguous eighths, 128^3 63.1 us against 71.9, 96x200x100 58.7 against
+  // 66.7, 256x128x128 123.6 against 130.1 (one global order 117.3).
+  p->xcd_remap = n == 2 ? 1 : 2;
   p->variant = FACES_VARIANT_AUTO;
   p->lds_block = 256;
   // Blocks: 2 persistent blocks per CU for n = 6-7 (60^3 n=7 276 us against
@@ -841,8 +843,10 @@ int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, i
   // 40^3 n=11 280 / 275 / 267; 128^3 n=4 412 / 426 / 411; 100^3 n=5 413 /
   // 413 / 412; 64^3 n=8 409 / 415 / 408; but 160^3 n=3 494 / 482 / 527,
   // 44^3 n=10 266.5 / 281 / 272, 40^3 n=12 358 / 416 / 399, 256^3 n=2 728 /
-  // 745 / 763.
-  static const int kFlowDefault[13] = {0, 0, 2, 3, 4, 4, 4, 4, 4, 4, 2, 4, 2};
+  // 745 / 763.  n=2 takes 8-byte lanes (flow 1), with contiguous eighths
+  // (us, flows 1 / 2 / 3 / 4): 128^3 63.1 / 67.8 / 66.4 / 69.1, 96x200x100
+  // 58.7 / 63.1 / 61.9 / 64.7.
+  static const int kFlowDefault[13] = {0, 0, 1, 3, 4, 4, 4, 4, 4, 4, 2, 4, 2};
   p->flow = (n >= 2 && n <= 12) ? kFlowDefault[n] : 2;
   // staged step, 60^3 n=7, interior rank of 3x3x3: surface blocks after the
   // items 259.7 us, before them 261, a launch of their own 267.5, summed by

@@ -145,11 +145,11 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *   FACES_OPT_VARIANT        FACES_VARIANT_*
  *   FACES_OPT_DIRECT_BLOCK   threads per block of the DIRECT kernel
  *   FACES_OPT_SEG            elements per work item (0 = default)
- *   FACES_OPT_XCD            XCD work order: 2 (default) = each XCD takes an
- *                            eighth of every plane's items (the one-item
- *                            kernel; the others read it as 1), 1 =
- *                            contiguous eighths of the work list, 0 = one
- *                            global order
+ *   FACES_OPT_XCD            XCD work order: 2 (default but for n = 2) =
+ *                            each XCD takes an eighth of every plane's items
+ *                            (the one-item kernel; the others read it as 1),
+ *                            1 (default for n = 2) = contiguous eighths of
+ *                            the work list, 0 = one global order
  *   FACES_OPT_LDS_BLOCK      threads per block of the LDS kernel
  *   FACES_OPT_STREAM_BPC     STREAM kernel: persistent blocks per CU (1-32);
  *                            0 = one work unit per block; -2, -3, -4 = one
@@ -178,14 +178,15 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *                            4 GB): the one-item kernel, one work item per
  *                            block in memory order, with load widths 2 = OWN
  *                            runs loaded 16 bytes per lane (8 when the field
- *                            is not 16-byte aligned; default for n = 2, 10,
+ *                            is not 16-byte aligned; default for n = 10,
  *                            12); 3 = loading and storing exactly the
  *                            locations each item owns, in 16-byte pairs that
  *                            may start on any double (default for n = 3); 4
  *                            = the same for the partner regions too (default
  *                            for n = 4-9 and 11; the staged step's surface
  *                            sums done by its items, FACES_OPT_SURFACE_BLOCKS
- *                            3, take 3); 1 = 8-byte lanes; 0 = the STREAM
+ *                            3, take 3); 1 = 8-byte lanes (default for n =
+ *                            2); 0 = the STREAM
  *                            walks above
  *   FACES_OPT_SURFACE_BLOCKS the staged step's surface sums where its interior
  *                            sums take the one-item kernel: 2 (default) =
