@@ -292,7 +292,7 @@ bool stream_case(int n, int S)
     case 6: return n <= 7;
     case 8: return n <= 6;
     case 16: return n <= 4;
-    case 32: return n == 2;
+    case 32: return n <= 3;
     default: return false;
   }
 }
@@ -326,7 +326,8 @@ static int item_len(int n, int want, int cols, bool divides)
 
 // Elements per item of the one-item kernel, with its own defaults per n
 // (60^3 n=7: 5 elements 249-253 us, 4 elements 254-256 us, 6 elements
-// 251-252 us).  Where the longest item dividing mx is shorter than the
+// 251-252 us; n=3, 32 elements against 16: 160^3 421 us against 483, 128^3
+// 210 against 252, 96x100x200 203 against 232).  Where the longest item dividing mx is shorter than the
 // longest that fits by 3 or more elements (61 columns, n=7: 1 against 5),
 // items of the longer length cover the first mx - mx % S columns and a
 // second launch of items of flow_rest elements the rest: each element owns
@@ -338,7 +339,7 @@ static int item_len(int n, int want, int cols, bool divides)
 int flow_seg(const faces_plan *p)
 {
   if (p->n < 2 || p->n > 12) return 0;
-  static const int kDefault[13] = {0, 0, 32, 16, 16, 10, 5, 5, 4, 3, 2, 1, 1};
+  static const int kDefault[13] = {0, 0, 32, 32, 16, 10, 5, 5, 4, 3, 2, 1, 1};
   const int want = p->seg_len > 0 ? p->seg_len : kDefault[p->n];
   const int sd = item_len(p->n, want, p->mx, true), sw = item_len(p->n, want, p->mx, false);
   return sw >= sd + 3 ? sw : sd;
@@ -503,7 +504,7 @@ int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, int tre
   if constexpr (N <= 4) {
     if (S == 16) return launch_stream_ns<N, 16>(p, G, mode, tree, W, st, jxb, cols);
   }
-  if constexpr (N == 2) {
+  if constexpr (N <= 3) {
     if (S == 32) return launch_stream_ns<N, 32>(p, G, mode, tree, W, st, jxb, cols);
   }
   return fail(FACES_ERR_ARG, "streaming kernel has no n=%d S=%d specialisation", N, S);
@@ -845,8 +846,10 @@ int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, i
   // 44^3 n=10 266.5 / 281 / 272, 40^3 n=12 358 / 416 / 399, 256^3 n=2 728 /
   // 745 / 763.  n=2 takes 8-byte lanes (flow 1), with contiguous eighths
   // (us, flows 1 / 2 / 3 / 4): 128^3 63.1 / 67.8 / 66.4 / 69.1, 96x200x100
-  // 58.7 / 63.1 / 61.9 / 64.7.
-  static const int kFlowDefault[13] = {0, 0, 1, 3, 4, 4, 4, 4, 4, 4, 2, 4, 2};
+  // 58.7 / 63.1 / 61.9 / 64.7.  n=3 with its 32-element items (flows 1 / 2 /
+  // 3 / 4): 160^3 478.7 / 442.0 / 424.5 / 421.2, 128^3 245.7 / 222.2 / 215.0
+  // / 209.7.
+  static const int kFlowDefault[13] = {0, 0, 1, 4, 4, 4, 4, 4, 4, 4, 2, 4, 2};
   p->flow = (n >= 2 && n <= 12) ? kFlowDefault[n] : 2;
   // staged step, 60^3 n=7, interior rank of 3x3x3: surface blocks after the
   // items 259.7 us, before them 261, a launch of their own 267.5, summed by

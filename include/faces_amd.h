@@ -181,9 +181,9 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *                            is not 16-byte aligned; default for n = 10,
  *                            12); 3 = loading and storing exactly the
  *                            locations each item owns, in 16-byte pairs that
- *                            may start on any double (default for n = 3); 4
- *                            = the same for the partner regions too (default
- *                            for n = 4-9 and 11; the staged step's surface
+ *                            may start on any double; 4 = the same for the
+ *                            partner regions too (default for n = 3-9 and
+ *                            11; the staged step's surface
  *                            sums done by its items, FACES_OPT_SURFACE_BLOCKS
  *                            3, take 3); 1 = 8-byte lanes (default for n =
  *                            2); 0 = the STREAM
