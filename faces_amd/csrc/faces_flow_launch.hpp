@@ -100,6 +100,9 @@ int launch_flow_n(int S, int lw, int mode, int tree, const Geom &G, const FlowIt
   if constexpr (N <= 3) {
     if (S == 32) return launch_flow_ns<N, 32>(lw, mode, tree, G, F, nb, W, st);
   }
+  if constexpr (N == 2) {
+    if (S == 64) return launch_flow_ns<N, 64>(lw, mode, tree, G, F, nb, W, st);
+  }
   return fail(FACES_ERR_ARG, "one-item kernel has no n=%d S=%d specialisation", N, S);
 }
 

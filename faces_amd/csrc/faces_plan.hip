@@ -293,6 +293,7 @@ bool stream_case(int n, int S)
     case 8: return n <= 6;
     case 16: return n <= 4;
     case 32: return n <= 3;
+    case 64: return n == 2;
     default: return false;
   }
 }
@@ -305,7 +306,7 @@ int stream_seg(const faces_plan *p)
   if (p->n < 2 || p->n > 12) return 0;
   static const int kDefault[13] = {0, 0, 32, 16, 16, 10, 5, 4, 4, 3, 2, 1, 1};
   const int want = p->seg_len > 0 ? p->seg_len : kDefault[p->n];
-  for (int s = std::min(want, 32); s >= 1; s--) {
+  for (int s = std::min(want, 64); s >= 1; s--) {
     if (!stream_case(p->n, s) || p->mx % s) continue;
     if (stream_slots(p->n, s) <= kStreamMaxSlots) return s;
   }
@@ -317,7 +318,7 @@ int stream_seg(const faces_plan *p)
 // them; 0 if none.
 static int item_len(int n, int want, int cols, bool divides)
 {
-  for (int s = std::min(std::min(want, 32), cols); s >= 1; s--) {
+  for (int s = std::min(std::min(want, 64), cols); s >= 1; s--) {
     if (!stream_case(n, s) || (divides && cols % s)) continue;
     if (stream_slots(n, s) <= kStreamMaxSlots) return s;
   }
@@ -327,7 +328,9 @@ static int item_len(int n, int want, int cols, bool divides)
 // Elements per item of the one-item kernel, with its own defaults per n
 // (60^3 n=7: 5 elements 249-253 us, 4 elements 254-256 us, 6 elements
 // 251-252 us; n=3, 32 elements against 16: 160^3 421 us against 483, 128^3
-// 210 against 252, 96x100x200 203 against 232).  Where the longest item dividing mx is shorter than the
+// 210 against 252, 96x100x200 203 against 232, 64 elements 451 and 222 us;
+// n=2, 64 elements against 32: 128^3 50.5 us against 71.5, 256x128x128 94.6
+// against 137.2).  Where the longest item dividing mx is shorter than the
 // longest that fits by 3 or more elements (61 columns, n=7: 1 against 5),
 // items of the longer length cover the first mx - mx % S columns and a
 // second launch of items of flow_rest elements the rest: each element owns
@@ -339,7 +342,7 @@ static int item_len(int n, int want, int cols, bool divides)
 int flow_seg(const faces_plan *p)
 {
   if (p->n < 2 || p->n > 12) return 0;
-  static const int kDefault[13] = {0, 0, 32, 32, 16, 10, 5, 5, 4, 3, 2, 1, 1};
+  static const int kDefault[13] = {0, 0, 64, 32, 16, 10, 5, 5, 4, 3, 2, 1, 1};
   const int want = p->seg_len > 0 ? p->seg_len : kDefault[p->n];
   const int sd = item_len(p->n, want, p->mx, true), sw = item_len(p->n, want, p->mx, false);
   return sw >= sd + 3 ? sw : sd;
@@ -506,6 +509,9 @@ int launch_stream_n(const faces_plan *p, const Geom &G, int S, int mode, int tre
   }
   if constexpr (N <= 3) {
     if (S == 32) return launch_stream_ns<N, 32>(p, G, mode, tree, W, st, jxb, cols);
+  }
+  if constexpr (N == 2) {
+    if (S == 64) return launch_stream_ns<N, 64>(p, G, mode, tree, W, st, jxb, cols);
   }
   return fail(FACES_ERR_ARG, "streaming kernel has no n=%d S=%d specialisation", N, S);
 }
@@ -805,10 +811,9 @@ int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, i
   // XCD order: each XCD takes an eighth of every plane's items (the one-item
   // kernel; 64^3 n=8 410 us against 423 for contiguous eighths of the list,
   // 80^3 n=6 356 against 397); the other kernels read it as 1, contiguous
-  // eighths of their work list.  n=2 (64-byte elements, 8-byte lanes):
-  // contiguous eighths, 128^3 63.1 us against 71.9, 96x200x100 58.7 against
-  // 66.7, 256x128x128 123.6 against 130.1 (one global order 117.3).
-  p->xcd_remap = n == 2 ? 1 : 2;
+  // eighths of their work list (n=2, 64-element items: 128^3 50.5 us
+  // against 53.0, 256x128x128 94.6 against 120.7).
+  p->xcd_remap = 2;
   p->variant = FACES_VARIANT_AUTO;
   p->lds_block = 256;
   // Blocks: 2 persistent blocks per CU for n = 6-7 (60^3 n=7 276 us against
@@ -844,9 +849,8 @@ int plan_init(faces_plan *p, int rank, int lx, int ly, int lz, int mx, int my, i
   // 40^3 n=11 280 / 275 / 267; 128^3 n=4 412 / 426 / 411; 100^3 n=5 413 /
   // 413 / 412; 64^3 n=8 409 / 415 / 408; but 160^3 n=3 494 / 482 / 527,
   // 44^3 n=10 266.5 / 281 / 272, 40^3 n=12 358 / 416 / 399, 256^3 n=2 728 /
-  // 745 / 763.  n=2 takes 8-byte lanes (flow 1), with contiguous eighths
-  // (us, flows 1 / 2 / 3 / 4): 128^3 63.1 / 67.8 / 66.4 / 69.1, 96x200x100
-  // 58.7 / 63.1 / 61.9 / 64.7.  n=3 with its 32-element items (flows 1 / 2 /
+  // 745 / 763.  n=2 takes 8-byte lanes (flow 1; 64-element items, flows 1
+  // / 3: 128^3 50.5 / 54.3 us, 256x128x128 94.6 / 103.3).  n=3 with its 32-element items (flows 1 / 2 /
   // 3 / 4): 160^3 478.7 / 442.0 / 424.5 / 421.2, 128^3 245.7 / 222.2 / 215.0
   // / 209.7.
   static const int kFlowDefault[13] = {0, 0, 1, 4, 4, 4, 4, 4, 4, 4, 2, 4, 2};
@@ -1064,14 +1068,14 @@ int faces_plan_tune(faces_plan *p, double *u, int reps, void *stream, double *ou
   std::vector<Cand> cands;
   const int seg0 = p->seg_len;
   if (stream_ok(p) && !stream_big(p)) {
-    // the one-item kernel: the three longest item lengths, both loads, both
-    // XCD orders
+    // the one-item kernel: the three longest item lengths, every load width,
+    // both XCD orders
     std::vector<int> segs;
-    for (int sl = 32; sl >= 1 && segs.size() < 3; sl--) {
+    for (int sl = 64; sl >= 1 && segs.size() < 3; sl--) {
       p->seg_len = sl;
       if (flow_seg(p) == sl) segs.push_back(sl);
     }
-    for (int fl : {3, 2, 1})
+    for (int fl : {4, 3, 2, 1})
       for (int sl : segs)
         for (int x : {2, 1}) cands.push_back({FACES_VARIANT_STREAM, p->stream_walk, p->stream_rows, sl, p->stream_bpc, fl, x});
   }

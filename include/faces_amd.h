@@ -145,11 +145,11 @@ int faces_plan_set_variant(faces_plan *plan, int variant, int block);
  *   FACES_OPT_VARIANT        FACES_VARIANT_*
  *   FACES_OPT_DIRECT_BLOCK   threads per block of the DIRECT kernel
  *   FACES_OPT_SEG            elements per work item (0 = default)
- *   FACES_OPT_XCD            XCD work order: 2 (default but for n = 2) =
- *                            each XCD takes an eighth of every plane's items
- *                            (the one-item kernel; the others read it as 1),
- *                            1 (default for n = 2) = contiguous eighths of
- *                            the work list, 0 = one global order
+ *   FACES_OPT_XCD            XCD work order: 2 (default) = each XCD takes an
+ *                            eighth of every plane's items (the one-item
+ *                            kernel; the others read it as 1), 1 =
+ *                            contiguous eighths of the work list, 0 = one
+ *                            global order
  *   FACES_OPT_LDS_BLOCK      threads per block of the LDS kernel
  *   FACES_OPT_STREAM_BPC     STREAM kernel: persistent blocks per CU (1-32);
  *                            0 = one work unit per block; -2, -3, -4 = one

@@ -130,7 +130,7 @@ def test_stream_walks_bitexact(torch_cuda, walk, rows, seg, bpc, xcd, shape):
 
 FLOW_SHAPES = [(7, 12, 6, 3), (7, 10, 3, 2), (8, 4, 3, 5), (5, 10, 2, 3), (5, 6, 1, 2), (4, 16, 4, 1),
                (6, 5, 2, 2), (2, 32, 2, 1), (3, 16, 3, 2), (9, 3, 2, 2), (10, 2, 3, 1), (11, 2, 1, 2),
-               (12, 1, 2, 2), (7, 1, 1, 1), (7, 15, 9, 2), (3, 64, 2, 2)]
+               (12, 1, 2, 2), (7, 1, 1, 1), (7, 15, 9, 2), (3, 64, 2, 2), (2, 128, 2, 1)]
 
 
 @pytest.mark.parametrize("flow", [1, 2, 3, 4])
@@ -849,9 +849,10 @@ def test_plan_tune_then_bitexact(torch_cuda, shape, boundary):
 # mx - mx % S columns, a second launch the rest (n=7 mx=11: 2 items of 5 and
 # one of 1 per row; mx=13: 2 of 5, one of 3; n=5 mx=23: 2 of 10, one of 3;
 # n=8 mx=7: one of 4, one of 3; n=6 mx=7: one of 5, one of 2; n=4 mx=17: one
-# of 16, one of 1; n=2 and n=3 mx=35: one of 32, one of 3).
+# of 16, one of 1; n=2 and n=3 mx=35: one of 32, one of 3; n=2 mx=131: two
+# of 64, one of 3).
 SPLIT_SHAPES = [(7, 11, 3, 2), (7, 13, 2, 3), (5, 23, 2, 2), (8, 7, 3, 2), (6, 7, 2, 2), (4, 17, 2, 2),
-                (2, 35, 2, 1), (3, 35, 2, 2)]
+                (2, 35, 2, 1), (3, 35, 2, 2), (2, 131, 2, 2)]
 
 
 @pytest.mark.parametrize("flow", [1, 2, 3, 4])
