@@ -289,9 +289,11 @@ int faces_batch_share_per_grid(faces_batch *batch, int compute, void *stream);
 
 /* Launch shape: threads per block (0 or 256), longest work item in elements
  * (1-64), blocks per CU (1-256: each block takes one contiguous chunk of the
- * work list of equal estimated cost; -1: one block per work item, the
- * items of XCD x's eighth of the list dispatched in order); 0 keeps the
- * current value. */
+ * work list of equal estimated cost; -1: one block per work item, each XCD
+ * taking a contiguous share of the list of equal estimated cost, its items
+ * dispatched in order); 0 keeps the current value.  The grids are dealt
+ * into one bin per XCD by item cost, so every share holds the same mix of
+ * item sizes. */
 int faces_batch_set_tuning(faces_batch *batch, int block, int seg_len, int blocks_per_cu);
 
 /* Times every candidate launch shape on the bound fields (reps replays each,
