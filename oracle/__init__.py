@@ -1,8 +1,8 @@
 """ctypes wrapper of the CPU oracle (faces_oracle.c).
 
 Test infrastructure: imported only by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py.  It is the checker, never the thing measured or
-shipped.  Build with ``make -C oracle``.
+cpu_baseline leg of bench.py.  It checks results; it is never the thing
+measured or shipped.  Build with ``make -C oracle``.
 """
 from __future__ import annotations
 

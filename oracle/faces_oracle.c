@@ -1,8 +1,8 @@
 /*
  * faces_oracle.c -- CPU oracle for the Faces exchange + DSS ("share") step.
  *
- * THIS FILE IS TEST INFRASTRUCTURE.  It is the checker that the GPU library
- * (libfaces_amd.so) is compared against, and the timed CPU baseline of the
+ * THIS FILE IS TEST INFRASTRUCTURE.  It is the reference result the GPU
+ * library (libfaces_amd.so) is compared against, and the timed CPU baseline of the
  * benchmark.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
  * leg may load it.  No product code path calls into it.
  *
