@@ -625,11 +625,11 @@ struct SurfWork {
 // ---------------------------------------------------------------------------
 // One launch, its blocks cut into ranges computed on the host: one range per
 // face message and one for the 20 edge and corner messages.  A block of a
-// y- or z-face range copies kPackFaceEntries consecutive entries (runs of n
-// or n^2 contiguous doubles in the field); a block of an x-face range stages
-// `xelem` whole boundary elements in LDS with contiguous loads and picks
-// their ix = 0 (or n-1) column, whose doubles lie n apart in the field (for
-// elements too large for LDS, xelem = 0: entries straight from the field);
+// face range copies kPackFaceEntries consecutive entries (y and z faces:
+// runs of n or n^2 contiguous doubles in the field; x faces: the ix = 0 or
+// n-1 column, doubles n apart, read straight from the field -- measured
+// faster than staging whole elements in LDS to read them contiguously, 60^3
+// n=7 8.0 us against 9.2, 40^3 n=12 11.4 against 14.8, equal for n <= 5);
 // edge and corner entries find their slot among the 20 small ones.
 // dst[k] is the arena offset entry 0 of slot k goes to: the send slot k, or
 // -- fusing the self-halo exchange -- the receive slot opposite(k); -1 skips
@@ -641,7 +641,6 @@ struct PackDesc {
   long src_off[27];  // prefix sums of slot sizes (entries)
   long dst[26];
   int bstart[8];     // block prefix: ranges 0..5 the faces, 6 the edges and corners
-  int xelem;         // elements per block of an x-face range (0: no LDS staging)
 };
 
 // Field index of entry e of slot k (the reference's pack maps, Faces.cc:243-293).
@@ -695,8 +694,6 @@ __device__ __forceinline__ long pack_index(const Geom &G, int k, long e)
 template <int N>
 __global__ __launch_bounds__(256) void k_pack(Geom G, const PackDesc P, double *__restrict__ arena)
 {
-  const int n = nval<N>(G.n), n2 = n * n;
-  const long n3 = (long)n * n2;
   const double *__restrict__ u = G.u;
   const int b = blockIdx.x, tid = threadIdx.x;
   int r = 0;
@@ -706,26 +703,6 @@ __global__ __launch_bounds__(256) void k_pack(Geom G, const PackDesc P, double *
     const long dk = P.dst[r];
     if (dk < 0) return;
     const long cnt = P.src_off[r + 1] - P.src_off[r];
-    if (r < 2 && P.xelem > 0) {
-      // x face: whole elements through LDS, contiguous loads
-      extern __shared__ double lds[];
-      const int s = r & 1;
-      const int jx = s ? G.mx - 1 : 0, ix = s ? n - 1 : 0;
-      const int nel = G.my * G.mz;
-      const int e0 = lb * P.xelem, ecnt = min(P.xelem, nel - e0);
-      const int tot = ecnt * (int)n3;
-      for (int t = tid; t < tot; t += 256) {
-        const int e = t / (int)n3, q = t - e * (int)n3;
-        const int el = e0 + e, jy = el % G.my, jz = el / G.my;
-        lds[t] = u[jx * n3 + jy * G.ys + jz * G.ps + q];
-      }
-      __syncthreads();
-      for (int t = tid; t < ecnt * n2; t += 256) {
-        const int e = t / n2, q = t - e * n2;  // q = iy + n*iz
-        arena[dk + (long)(e0 + e) * n2 + q] = lds[e * (int)n3 + q * n + ix];
-      }
-      return;
-    }
 #pragma unroll
     for (int it = 0; it < kPackFaceEntries / 256; it++) {
       const long e = (long)lb * kPackFaceEntries + it * 256 + tid;

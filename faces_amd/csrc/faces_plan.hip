@@ -677,24 +677,18 @@ int launch_pack(const faces_plan *p, const double *u, bool direct, hipStream_t s
       P.dst[k] = (p->nbr[o] >= 0) ? p->recv.off[o] : -1;
     }
   }
-  // x faces: as many whole elements per block as fit 32 KiB of LDS (up to 8)
-  const long n3 = (long)p->n * p->n * p->n;
-  P.xelem = (int)std::min(8L, (32L * 1024 / 8) / n3);
-  const long nel_x = (long)p->my * p->mz;
   long nb = 0;
   for (int r = 0; r < 6; r++) {
     P.bstart[r] = (int)nb;
     if (P.dst[r] < 0) continue;  // a skipped slot takes no blocks
-    nb += (r < 2 && P.xelem > 0) ? (nel_x + P.xelem - 1) / P.xelem
-                                 : (p->slot_cnt[r] + kPackFaceEntries - 1) / kPackFaceEntries;
+    nb += (p->slot_cnt[r] + kPackFaceEntries - 1) / kPackFaceEntries;
   }
   P.bstart[6] = (int)nb;
   nb += (P.src_off[26] - P.src_off[6] + 255) / 256;
   P.bstart[7] = (int)nb;
   if (nb > 0x7fffffffL / 2) return fail(FACES_ERR_ARG, "pack launch of %ld blocks", nb);
-  const size_t lds = P.xelem > 0 ? (size_t)P.xelem * n3 * sizeof(double) : 0;
   const dim3 grid((unsigned)nb), blk(256);
-#define FACES_LAUNCH_PACK(NN) k_pack<NN><<<grid, blk, lds, st>>>(G, P, p->arena);
+#define FACES_LAUNCH_PACK(NN) k_pack<NN><<<grid, blk, 0, st>>>(G, P, p->arena);
   FACES_DISPATCH_N(p->n, FACES_LAUNCH_PACK)
 #undef FACES_LAUNCH_PACK
   FACES_HIP(hipGetLastError());
