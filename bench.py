@@ -47,6 +47,8 @@ CONFIGS = {
     "c3": (8, 32, 32, 32, 1, "one rank of 6x6x6 periodic self-halo, 32x32x32 elements, n=8, "
                             "random [-1,1) seed 42"),
     "c1": (5, 4, 4, 4, 0, "plain step 1x1x1 periodic, 4x4x4 elements, n=5, integer seed 42"),
+    "ragged": (7, 61, 60, 60, 0, "plain step 1x1x1 periodic, 61x60x60 elements (rows of 12 five-element "
+                                "items and one of one element), n=7, integer-valued [0,1023] seed 42"),
 }
 
 
