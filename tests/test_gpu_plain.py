@@ -880,7 +880,7 @@ def test_flow_split_rows_bitexact(torch_cuda, flow, xcd, shape):
 @pytest.mark.parametrize("path,sblocks", [(fa.PATH_FUSED, 2), (fa.PATH_STAGED_DIRECT, 2),
                                           (fa.PATH_STAGED_DIRECT, 1), (fa.PATH_STAGED_DIRECT, 3),
                                           (fa.PATH_STAGED_DIRECT, 0), (fa.PATH_STAGED, 2)])
-@pytest.mark.parametrize("shape", [(7, 11, 3, 2), (6, 7, 2, 2), (5, 23, 2, 2), (3, 35, 3, 2)])
+@pytest.mark.parametrize("shape", [(7, 11, 3, 2), (6, 7, 2, 2), (5, 23, 2, 2), (3, 35, 3, 2), (2, 70, 2, 2)])
 def test_split_rows_open_ranks(torch_cuda, path, sblocks, shape):
     """Open ranks of a 3x3x3 grid over split rows: the fused sweep, and the
     staged step with its surface sums as blocks of the first launch (after
@@ -905,7 +905,7 @@ def test_split_rows_open_ranks(torch_cuda, path, sblocks, shape):
         assert_same(got, want, shape)
 
 
-@pytest.mark.parametrize("shape", [(7, 11, 3, 2), (8, 7, 3, 2)])
+@pytest.mark.parametrize("shape", [(7, 11, 3, 2), (8, 7, 3, 2), (2, 70, 2, 2)])
 def test_split_rows_six(torch_cuda, shape):
     """The 6-message schedule in one launch per item length (tree z, y, x)
     over split rows: open ranks and the periodic step, bit-exact."""
