@@ -847,12 +847,13 @@ def test_plan_tune_then_bitexact(torch_cuda, shape, boundary):
 # (n, mx, my, mz) where the longest item dividing mx is 3 or more elements
 # shorter than the default item: items of the default length cover the first
 # mx - mx % S columns, a second launch the rest (n=7 mx=11: 2 items of 5 and
-# one of 1 per row; mx=13: 2 of 5, one of 3; n=5 mx=23: 2 of 10, one of 3;
-# n=8 mx=7: one of 4, one of 3; n=6 mx=7: one of 5, one of 2; n=4 mx=17: one
-# of 16, one of 1; n=2 and n=3 mx=35: one of 32, one of 3; n=2 mx=131: two
-# of 64, one of 3).
-SPLIT_SHAPES = [(7, 11, 3, 2), (7, 13, 2, 3), (5, 23, 2, 2), (8, 7, 3, 2), (6, 7, 2, 2), (4, 17, 2, 2),
-                (2, 35, 2, 1), (3, 35, 2, 2), (2, 131, 2, 2)]
+# one of 1 per row; n=8 mx=9: 2 of 4, one of 1; n=6 mx=11: 2 of 5, one of 1;
+# n=7 mx=13: 2 of 5, one of 3; n=5 mx=23: 2 of 10, one of 3; n=8 mx=7: one
+# of 4, one of 3; n=6 mx=7: one of 5, one of 2; n=4 mx=17: one of 16, one of
+# 1; n=2 and n=3 mx=35: one of 32, one of 3; n=2 mx=131: two of 64, one of
+# 3).
+SPLIT_SHAPES = [(7, 11, 3, 2), (8, 9, 3, 2), (6, 11, 2, 2), (7, 13, 2, 3), (5, 23, 2, 2), (8, 7, 3, 2),
+                (6, 7, 2, 2), (4, 17, 2, 2), (2, 35, 2, 1), (3, 35, 2, 2), (2, 131, 2, 2)]
 
 
 @pytest.mark.parametrize("flow", [1, 2, 3, 4])
@@ -880,7 +881,8 @@ def test_flow_split_rows_bitexact(torch_cuda, flow, xcd, shape):
 @pytest.mark.parametrize("path,sblocks", [(fa.PATH_FUSED, 2), (fa.PATH_STAGED_DIRECT, 2),
                                           (fa.PATH_STAGED_DIRECT, 1), (fa.PATH_STAGED_DIRECT, 3),
                                           (fa.PATH_STAGED_DIRECT, 0), (fa.PATH_STAGED, 2)])
-@pytest.mark.parametrize("shape", [(7, 11, 3, 2), (6, 7, 2, 2), (5, 23, 2, 2), (3, 35, 3, 2), (2, 70, 2, 2)])
+@pytest.mark.parametrize("shape", [(7, 11, 3, 2), (6, 7, 2, 2), (5, 23, 2, 2), (3, 35, 3, 2), (2, 70, 2, 2),
+                                   (8, 9, 3, 2)])
 def test_split_rows_open_ranks(torch_cuda, path, sblocks, shape):
     """Open ranks of a 3x3x3 grid over split rows: the fused sweep, and the
     staged step with its surface sums as blocks of the first launch (after
@@ -905,7 +907,7 @@ def test_split_rows_open_ranks(torch_cuda, path, sblocks, shape):
         assert_same(got, want, shape)
 
 
-@pytest.mark.parametrize("shape", [(7, 11, 3, 2), (8, 7, 3, 2), (2, 70, 2, 2)])
+@pytest.mark.parametrize("shape", [(7, 11, 3, 2), (8, 7, 3, 2), (2, 70, 2, 2), (8, 9, 3, 2)])
 def test_split_rows_six(torch_cuda, shape):
     """The 6-message schedule in one launch per item length (tree z, y, x)
     over split rows: open ranks and the periodic step, bit-exact."""
